@@ -1,0 +1,212 @@
+"""Throughput benchmark of the PAR-SGNS trainer on one MI355X.
+
+Workload (default): the 1B-word-shaped corpus -- 800M Zipf(1.0) tokens over
+555,000 words, log-normal(3.0, 0.8) line lengths clipped to [5, 2000], 1000
+planted clusters of 20 words (p = 0.2), d 300, window 5, 10 negatives,
+sample 1e-5, one epoch cut into ~10M-raw-word chunks.  A step is one chunk:
+one launch of the training kernel over the sentences of that chunk (a
+captured hipGraph for the persistent variant), inputs resident in HBM.
+
+Both variants train the same epoch from the same InitNet weights:
+  fp32 plain      one wavefront per sentence, fp32 tables, float atomics
+  bf16 persistent work-queue kernel, bf16 tables, fp32 math, stochastic
+                  rounding, hipGraph per chunk
+`value` is the bf16 persistent rate in Mwords/s: raw corpus words (every
+token incl. those removed by subsampling and "</s>") per second of the timed
+steps.  The planted-cluster recall@10 of both runs is reported beside it.
+
+The CPU baseline is the oracle's OpenMP Hogwild restatement of the
+reference CPU trainer, timed on a bounded prefix of the same corpus."""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from par_word2vec_amd import corpora, host as H  # noqa: E402
+from par_word2vec_amd.device import SgnsDevice  # noqa: E402
+from par_word2vec_amd.trainer import chunk_bounds  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(tokens: int, negatives: int, dim: int, elem: int) -> float:
+    """4 B token id per trained token + each trained token's W_in and W_out
+    rows read and written once + each shared negative's W_out row read and
+    written once."""
+    return 4.0 * tokens + elem * dim * (4.0 * tokens + 2.0 * negatives)
+
+
+def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int, steps: int, csr, seed: int,
+                profile_only: bool = False) -> dict:
+    w = c.workload
+    V = len(c.vocab)
+    dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision)
+    dev.set_exp_table(H.exp_table())
+    w_in, w_out = H.init_net(V, w.dim)
+    dev.set_weights(w_in, w_out)
+    dev.set_sampler(H.unigram_entries(c.counts))
+    dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
+    n = len(chunks)
+    use_graph = mode == "persistent"
+
+    def step(i: int) -> dict:
+        b, e = chunks[i % n]
+        return dev.train(mode, b, e, w.alpha, c.train_words, iters=w.iters, epoch=0, seed=seed, use_graph=use_graph)
+
+    for i in range(warmup):
+        step(i)
+    log(f"{mode}/{precision}: warmup {warmup} steps done")
+    t0 = time.perf_counter()
+    stats = []
+    raw = 0
+    for i in range(warmup, warmup + steps):
+        stats.append(step(i))
+        raw += raw_per_chunk[i % n]
+        if (i - warmup) % 20 == 19:
+            log(f"{mode}/{precision}: step {i - warmup + 1}/{steps}")
+    wall = time.perf_counter() - t0
+    done = warmup + steps
+    for i in range(done, n):  # finish the epoch (untimed) so quality is scored on a full pass
+        step(i)
+    kern_ms = sum(s["kernel_ms"] for s in stats)
+    tokens = sum(s["tokens"] for s in stats)
+    negs = sum(s["negatives"] for s in stats)
+    elem = 2 if precision == "bf16" else 4
+    byts = algorithmic_bytes(tokens, negs, w.dim, elem)
+    out = dict(mode=mode, precision=precision, wall_s=wall, kernel_ms=kern_ms, raw_words=raw, tokens=tokens,
+               negatives=negs, pairs=sum(s["pairs"] for s in stats), mwords_s=raw / wall / 1e6,
+               kernel_mwords_s=raw / (kern_ms / 1e3) / 1e6, alg_bytes=byts, launches=len(stats),
+               avg_launch_ms=kern_ms / max(len(stats), 1), achieved_gbs=byts / (kern_ms / 1e3) / 1e9)
+    if not profile_only:
+        g_in, _ = dev.get_weights()
+        out["w_in"] = g_in
+    dev.close()
+    log(f"{mode}/{precision}: {out['mwords_s']:.1f} Mwords/s wall, {out['kernel_mwords_s']:.1f} kernel, "
+        f"{out['achieved_gbs']:.0f} GB/s algorithmic")
+    return out
+
+
+def cluster_score(c, w_in) -> float | None:
+    from par_word2vec_amd import evaluate as E
+    try:
+        return E.planted_cluster_recall(w_in, c.groups)
+    except Exception as ex:  # noqa: BLE001 - scoring must not hide the throughput line
+        log(f"cluster scoring failed: {ex!r}")
+        return None
+
+
+def cpu_baseline(c, csr, seconds: float) -> dict:
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    w = c.workload
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    # bounded sample: a prefix of the corpus sized to roughly `seconds` of CPU
+    # work, calibrated by a short probe run
+    table = O.unigram_table(c.counts)
+    probe = min(c.stream.size, 400_000)
+    cut = int(np.flatnonzero(c.stream[:probe] == 0)[-1]) + 1
+    secs, _, _ = O.train_hogwild(c.stream[:cut], c.counts, c.train_words, w.dim, w.window, w.negative,
+                                 w.batch_size, 1, w.alpha, w.sample, threads, table=table)
+    rate = cut / max(secs, 1e-6)
+    n = int(min(c.stream.size, max(cut, rate * seconds)))
+    ends = np.flatnonzero(c.stream[:n] == 0)
+    n = int(ends[-1]) + 1 if ends.size else n
+    secs, st, _ = O.train_hogwild(c.stream[:n], c.counts, c.train_words, w.dim, w.window, w.negative, w.batch_size,
+                                  1, w.alpha, w.sample, threads, table=table)
+    return {"value": round(n / secs / 1e6, 3), "unit": "Mwords/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} raw words ({st['trained_tokens']} trained tokens) of the same corpus, "
+                      f"1 epoch, {secs:.1f} s, OpenMP Hogwild restatement of PAR_Word2Vec_cpu.cpp"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed chunks (0 = rest of the epoch)")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="1b")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--chunk-words", type=int, default=10_000_000)
+    ap.add_argument("--variants", default="fp32-plain,bf16-persistent")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-score", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
+              f"(the multi-GPU path is a design note, not code)", file=sys.stderr)
+        return 2
+
+    t0 = time.perf_counter()
+    c = corpora.make(args.workload, args.scale)
+    w = c.workload
+    log(f"corpus {args.workload}: {c.stream.size} raw words, V={len(c.vocab)}, {time.perf_counter() - t0:.1f} s")
+    csr = H.subsample(c.stream, c.counts, c.train_words, w.sample, seed=args.seed * 1000003)
+    chunks = chunk_bounds(csr.sent_raw, args.chunk_words)
+    bounds_raw = [int(csr.sent_raw[b]) for b, _ in chunks] + [int(c.stream.size)]
+    raw_per_chunk = [bounds_raw[i + 1] - bounds_raw[i] for i in range(len(chunks))]
+    steps = args.steps if args.steps > 0 else max(1, len(chunks) - args.warmup)
+    log(f"{csr.n_sent} sentences, {csr.tokens.size} trained tokens, {len(chunks)} chunks; warmup {args.warmup}, "
+        f"steps {steps}")
+
+    results = {}
+    for v in args.variants.split(","):
+        prec, mode = v.split("-")
+        results[v] = run_variant(c, mode, prec, chunks, raw_per_chunk, args.warmup, steps, csr, args.seed,
+                                 profile_only=args.no_score)
+    scores = {}
+    if not args.no_score:
+        for v, r in results.items():
+            scores[v] = cluster_score(c, r.pop("w_in"))
+            log(f"{v}: planted-cluster recall@10 = {scores[v]}")
+    head = results.get("bf16-persistent") or next(iter(results.values()))
+    base = results.get("fp32-plain")
+    cpu = None if args.no_cpu else cpu_baseline(c, csr, args.cpu_seconds)
+    line = {
+        "metric": "Mwords/sec trained, BF16 persistent vs FP32 plain, at equal planted-cluster score",
+        "value": round(head["mwords_s"], 2),
+        "unit": "Mwords/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["wall_s"] * 1e3 / steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if head["precision"] == "bf16" else "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "raw_words": int(c.stream.size), "vocab": len(c.vocab), "dim": w.dim,
+                   "window": w.window, "negative": w.negative, "sample": w.sample, "batch_size": w.batch_size,
+                   "chunk_words": args.chunk_words, "chunks": len(chunks), "trained_tokens": int(csr.tokens.size),
+                   "variant": f"{head['precision']} {head['mode']}", "parallelism": "single GPU"},
+        "fp32_plain_mwords_s": round(base["mwords_s"], 2) if base else None,
+        "speedup_vs_fp32_plain": round(head["mwords_s"] / base["mwords_s"], 3) if base else None,
+        "cluster_recall_at_10": {k: (round(s, 4) if s is not None else None) for k, s in scores.items()},
+        "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
+                     for k, r in results.items()},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"sgns_fast_kernel<{'bf16' if head['precision'] == 'bf16' else 'f32'}>",
+                     "avg_launch_ms": round(head["avg_launch_ms"], 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

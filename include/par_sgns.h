@@ -1,0 +1,152 @@
+/*
+ * par_sgns.h -- device C-ABI of the MI355X PAR-SGNS trainer (libpar_sgns.so).
+ *
+ * Replaces the reference's host->device boundary of PAR_Word2Vec_gpu_text.cu:
+ * the device globals it owns (text.cu:84-86), their cudaMalloc/cudaMemcpy
+ * staging (text.cu:112-113, 371-374, 841-868, 963), and the launch of
+ * par_sgns_kernel (text.cu:381, 899, 912).  Instead of globals and exit(0) on
+ * error (text.cu:42-48), a handle owns all device memory and one HIP stream,
+ * every call returns a status code, and par_sgns_last_error() explains the
+ * last failure.  No torch or HIP types appear in the signatures.
+ *
+ * Data layout contract (PAR_Word2Vec_cpu.cpp:80, 338-339): W_in and W_out are
+ * row-major [vocab_size][dim] float32 on the host side of this interface; on
+ * the device each row is padded to `row_stride` elements (zeros) and stored in
+ * the configured precision.  Token id 0 is "</s>".
+ *
+ * One handle per host thread.  Calls on one handle are serialised on its
+ * stream; every call that returns results synchronises that stream.
+ */
+#ifndef PAR_SGNS_H
+#define PAR_SGNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PAR_SGNS_OK = 0,
+    PAR_SGNS_EINVAL = -1,   /* bad argument or unsupported shape */
+    PAR_SGNS_ENOMEM = -2,   /* device allocation failed */
+    PAR_SGNS_EDEVICE = -3,  /* HIP runtime error */
+    PAR_SGNS_ESTATE = -4,   /* call out of order (e.g. train before upload) */
+};
+
+enum par_sgns_precision { PAR_SGNS_FP32 = 0, PAR_SGNS_BF16 = 1 };
+
+enum par_sgns_mode {
+    PAR_SGNS_DETERMINISTIC = 0, /* one wavefront replays an uploaded plan exactly */
+    PAR_SGNS_PLAIN = 1,         /* one workgroup (one wavefront) per sentence */
+    PAR_SGNS_PERSISTENT = 2,    /* resident wavefronts pull sentences from a queue */
+};
+
+typedef struct par_sgns_handle_s *par_sgns_handle;
+
+typedef struct par_sgns_config {
+    int32_t vocab_size;   /* V, including "</s>" */
+    int32_t dim;          /* -size, 1..512 */
+    int32_t window;       /* -window, 1..64 */
+    int32_t negative;     /* -negative, 0..64 */
+    int32_t batch_size;   /* -batch-size (repulsion batch B), 1..64 */
+    int32_t precision;    /* par_sgns_precision */
+    int32_t device;       /* HIP device ordinal */
+    int32_t reserved;
+} par_sgns_config;
+
+typedef struct par_sgns_train_args {
+    int32_t mode;         /* par_sgns_mode */
+    int32_t iter;         /* total epochs of the schedule (alpha decay) */
+    int32_t epoch;        /* current epoch, 0-based */
+    int32_t compute_loss; /* accumulate the FP64 SGNS loss */
+    float alpha;          /* starting learning rate alpha0 */
+    int32_t waves_per_block; /* persistent mode; 0 = tuned default */
+    int32_t blocks;          /* persistent mode; 0 = tuned default */
+    int32_t use_graph;       /* persistent mode: replay a captured hipGraph */
+    uint64_t train_words; /* T = sum of vocabulary counts */
+    uint64_t seed;        /* Philox key for windows, negatives, rounding */
+    int64_t sent_begin;   /* sentence range [begin, end) of the uploaded corpus */
+    int64_t sent_end;     /* (deterministic mode: of the uploaded plan) */
+} par_sgns_train_args;
+
+typedef struct par_sgns_stats {
+    double loss_attr;     /* sum of -log sigmoid(f), attraction pairs */
+    double loss_rep;      /* sum of -log sigmoid(-f), repulsion pairs */
+    uint64_t tokens;      /* centre tokens trained */
+    uint64_t sentences;
+    uint64_t pairs;       /* attraction pairs */
+    uint64_t batches;     /* repulsion batches */
+    uint64_t negatives;   /* sum over batches of K */
+    uint64_t rep_pairs;   /* sum over batches of K * batch rows */
+    double kernel_ms;     /* device time of the training launches (HIP events) */
+} par_sgns_stats;
+
+/* Handle lifetime.  Replaces the device allocations of text.cu:348-376,
+ * 841-868 and the cudaFree calls of text.cu:969-977. */
+int par_sgns_create(const par_sgns_config *cfg, par_sgns_handle *out);
+int par_sgns_destroy(par_sgns_handle h);
+const char *par_sgns_last_error(par_sgns_handle h);
+/* Device row stride (elements) of W_in/W_out. */
+int32_t par_sgns_row_stride(par_sgns_handle h);
+
+/* expTable upload (text.cu:1092-1100); n must be 1001. */
+int par_sgns_set_exp_table(par_sgns_handle h, const float *table, int32_t n);
+
+/* Negative sampler: entries[V] = slots per word in the reference's unigram
+ * table (cpu:96-117).  Built into an on-device alias table; the table's
+ * "</s>" slots are folded into a uniform draw over [1, V-1] as cpu:606-608
+ * does.  Replaces the 400 MB d_table upload (text.cu:112-113). */
+int par_sgns_set_sampler(par_sgns_handle h, const int64_t *entries, int32_t vocab_size);
+
+/* Weights (text.cu:371-374 upload, text.cu:963 download).  Host arrays are
+ * [V][dim] float32; NULL skips that table.  BF16 handles round to nearest on
+ * upload. */
+int par_sgns_set_weights(par_sgns_handle h, const float *w_in, const float *w_out);
+int par_sgns_get_weights(par_sgns_handle h, float *w_in, float *w_out);
+
+/* Sentence CSR for the fast modes (d_corpus / d_sen_len_ptr, text.cu:841-845):
+ * tokens[n_tokens], sent_off[n_sent+1] (int64), sent_raw[n_sent] = raw-word
+ * offset of each sentence in the epoch (learning-rate schedule).  Sentences
+ * longer than 1000 tokens are rejected. */
+int par_sgns_set_corpus(par_sgns_handle h, const int32_t *tokens, int64_t n_tokens,
+                        const int64_t *sent_off, const int64_t *sent_raw, int64_t n_sent);
+
+/* Exact schedule for the deterministic mode (layout of ph_plan_copy in
+ * par_host.h). */
+int par_sgns_set_plan(par_sgns_handle h, const int32_t *tokens, const uint8_t *b, int64_t n_tokens,
+                      const int64_t *sent_off, const float *sent_alpha, int64_t n_sent,
+                      const int64_t *batch_off, const int64_t *neg_off, int64_t n_batches,
+                      const int32_t *negs, int64_t n_negs);
+
+/* One training pass over a sentence range (the epoch loop body of
+ * text.cu:879-952: one par_sgns_kernel launch per epoch there). */
+int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *args, par_sgns_stats *stats);
+
+/* SGNS window-batch loss and its adjoint (no counterpart in the reference,
+ * which fuses them into the update at cpu:551-565, 633-656).  Window w has
+ * centre[w] (W_out row, label 1), ctx[w*nctx..] (W_in rows) and shared
+ * negatives neg[w*nneg..] (W_out rows, label 0); the logits are
+ * F = X [c, n_1..n_nneg]^T per window (nctx <= 32, nneg <= 7).  loss_out
+ * receives the FP64 sum of -log sigmoid(F_i0) + sum_k -log sigmoid(-F_ik)
+ * with the exact sigmoid.  The adjoint writes dense gradients dL/dW_in and
+ * dL/dW_out (FP32, [V][dim], zeroed first) to the handle's gradient buffers,
+ * fetched with par_sgns_get_grads.  flags: PAR_SGNS_WIN_FP64 evaluates the
+ * logits and the loss in FP64 (gradient checks); PAR_SGNS_WIN_MFMA runs the
+ * (nctx x dim)(dim x (1+nneg)) contraction on BF16 matrix cores with FP32
+ * accumulation (BF16 tables only). */
+enum { PAR_SGNS_WIN_MFMA = 1, PAR_SGNS_WIN_FP64 = 2 };
+int par_sgns_loss_forward(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
+                          int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out);
+int par_sgns_loss_adjoint(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
+                          int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out);
+int par_sgns_get_grads(par_sgns_handle h, float *g_in, float *g_out);
+
+/* Device time (ms) and launch count of the most recent kernel of the given
+ * kind: 0 = training, 1 = loss forward, 2 = adjoint. */
+int par_sgns_last_kernel_time(par_sgns_handle h, int32_t kind, double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,617 @@
+// api.hip -- C-ABI of libpar_sgns.so (declared in include/par_sgns.h).
+//
+// A handle owns every device buffer, one HIP stream (default priority) and
+// two timing events.  All uploads validate indices on the host before any
+// kernel can see them, so a bad token id or offset becomes PAR_SGNS_EINVAL
+// instead of an out-of-bounds access on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../../include/par_sgns.h"
+#include "sgns_device.h"
+#include "sgns_internal.h"
+
+namespace {
+
+struct GraphKey {
+    int64_t b, e;
+    int32_t epoch, loss, wpb, blocks, iter;
+    uint32_t alpha_bits;
+    uint64_t seed, T;
+    bool operator<(const GraphKey &o) const {
+        return std::tie(b, e, epoch, loss, wpb, blocks, iter, alpha_bits, seed, T) <
+               std::tie(o.b, o.e, o.epoch, o.loss, o.wpb, o.blocks, o.iter, o.alpha_bits, o.seed, o.T);
+    }
+};
+
+}  // namespace
+
+struct par_sgns_handle_s {
+    par_sgns_config cfg{};
+    int ld = 0, nq = 0, np = 0, bf16 = 0, kmax = 0, R = 0;
+    int cu_count = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *w_in = nullptr, *w_out = nullptr;
+    float *exp_table = nullptr;
+    float *alias_prob = nullptr;
+    int32_t *alias_idx = nullptr;
+    bool have_exp = false, have_sampler = false, have_weights = false;
+    // fast-mode corpus
+    int32_t *tokens = nullptr;
+    int64_t *sent_off = nullptr, *sent_raw = nullptr;
+    int32_t *order = nullptr;
+    int64_t n_tokens = 0, n_sent = 0;
+    std::vector<int64_t> h_sent_off;
+    std::set<std::pair<int64_t, int64_t>> ordered;
+    // deterministic plan
+    int32_t *p_tokens = nullptr, *p_negs = nullptr;
+    uint8_t *p_b = nullptr;
+    int64_t *p_sent_off = nullptr, *p_batch_off = nullptr, *p_neg_off = nullptr;
+    float *p_alpha = nullptr;
+    float *det_scratch = nullptr;
+    int64_t p_n_sent = 0;
+    // counters
+    unsigned long long *counters = nullptr, *queue = nullptr;
+    double *loss = nullptr;
+    // window kernels
+    float *g_in = nullptr, *g_out = nullptr;
+    int32_t *win_buf = nullptr;
+    size_t win_cap = 0;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    double last_ms[3] = {0, 0, 0};
+    int last_launches[3] = {0, 0, 0};
+    std::string err;
+};
+
+namespace {
+
+int seterr(par_sgns_handle h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return seterr(h, e_ == hipErrorOutOfMemory ? PAR_SGNS_ENOMEM : PAR_SGNS_EDEVICE,     \
+                          "%s failed: %s", #call, hipGetErrorString(e_));                        \
+    } while (0)
+
+template <typename P>
+void dfree(P *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename P>
+int dalloc(par_sgns_handle h, P *&p, size_t n) {
+    dfree(p);
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)&p, n * sizeof(P));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return seterr(h, PAR_SGNS_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(P), hipGetErrorString(e));
+    }
+    return 0;
+}
+
+template <typename P>
+int upload(par_sgns_handle h, P *&p, const P *src, size_t n) {
+    int rc = dalloc(h, p, n);
+    if (rc) return rc;
+    if (n) HIPCHK(h, hipMemcpyAsync(p, src, n * sizeof(P), hipMemcpyHostToDevice, h->st));
+    return 0;
+}
+
+uint16_t f2bf_rne(float x) {
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+float bf2f(uint16_t v) {
+    uint32_t b = (uint32_t)v << 16;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+int check_ids(par_sgns_handle h, const int32_t *ids, int64_t n, int32_t lo, int32_t hi, const char *what) {
+    int64_t bad = -1;
+#pragma omp parallel for reduction(max : bad)
+    for (int64_t i = 0; i < n; i++)
+        if (ids[i] < lo || ids[i] >= hi) bad = std::max(bad, i);
+    if (bad >= 0) return seterr(h, PAR_SGNS_EINVAL, "%s[%lld] = %d outside [%d, %d)", what, (long long)bad, ids[bad], lo, hi);
+    return 0;
+}
+
+int check_offsets(par_sgns_handle h, const int64_t *off, int64_t n_sent, int64_t n_tokens, int max_len, const char *what) {
+    if (off[0] != 0) return seterr(h, PAR_SGNS_EINVAL, "%s[0] must be 0", what);
+    for (int64_t s = 0; s < n_sent; s++) {
+        int64_t L = off[s + 1] - off[s];
+        if (L < 0 || L > max_len)
+            return seterr(h, PAR_SGNS_EINVAL, "%s: sentence %lld has length %lld (allowed 0..%d)", what, (long long)s,
+                          (long long)L, max_len);
+    }
+    if (off[n_sent] != n_tokens) return seterr(h, PAR_SGNS_EINVAL, "%s[n_sent] != n_tokens", what);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int par_sgns_create(const par_sgns_config *cfg, par_sgns_handle *out) {
+    if (!cfg || !out) return PAR_SGNS_EINVAL;
+    *out = nullptr;
+    if (cfg->vocab_size < 2 || cfg->dim < 1 || cfg->dim > 512 || cfg->window < 1 || cfg->window > 64 ||
+        cfg->negative < 0 || cfg->negative > 64 || cfg->batch_size < 1 || cfg->batch_size > 64 ||
+        (cfg->precision != PAR_SGNS_FP32 && cfg->precision != PAR_SGNS_BF16))
+        return PAR_SGNS_EINVAL;
+    par_sgns_handle h = new par_sgns_handle_s;
+    h->cfg = *cfg;
+    h->bf16 = cfg->precision == PAR_SGNS_BF16;
+    h->ld = h->bf16 ? (cfg->dim + 63) / 64 * 64 : (cfg->dim + 31) / 32 * 32;
+    h->nq = (cfg->dim + 1) / 2;
+    h->np = (h->nq + 63) / 64;
+    h->R = 2 * cfg->window + 1;
+    h->kmax = (2 * cfg->window - 1) * cfg->negative;
+    if (h->kmax < 1) h->kmax = 1;
+    *out = h;
+    hipError_t e = hipSetDevice(cfg->device);
+    if (e != hipSuccess) return seterr(h, PAR_SGNS_EDEVICE, "hipSetDevice(%d): %s", cfg->device, hipGetErrorString(e));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreate(&h->ev0));
+    HIPCHK(h, hipEventCreate(&h->ev1));
+    HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, cfg->device));
+    const size_t rows = (size_t)cfg->vocab_size * (size_t)h->ld;
+    const size_t esz = h->bf16 ? 2 : 4;
+    int rc;
+    if ((rc = dalloc(h, reinterpret_cast<char *&>(h->w_in), rows * esz))) return rc;
+    if ((rc = dalloc(h, reinterpret_cast<char *&>(h->w_out), rows * esz))) return rc;
+    HIPCHK(h, hipMemsetAsync(h->w_in, 0, rows * esz, h->st));
+    HIPCHK(h, hipMemsetAsync(h->w_out, 0, rows * esz, h->st));
+    if ((rc = dalloc(h, h->counters, sgns::kCntSlots))) return rc;
+    if ((rc = dalloc(h, h->queue, 1))) return rc;
+    if ((rc = dalloc(h, h->loss, 2))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return 0;
+}
+
+int par_sgns_destroy(par_sgns_handle h) {
+    if (!h) return PAR_SGNS_EINVAL;
+    (void)hipSetDevice(h->cfg.device);
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    dfree(h->w_in); dfree(h->w_out); dfree(h->exp_table); dfree(h->alias_prob); dfree(h->alias_idx);
+    dfree(h->tokens); dfree(h->sent_off); dfree(h->sent_raw); dfree(h->order);
+    dfree(h->p_tokens); dfree(h->p_negs); dfree(h->p_b); dfree(h->p_sent_off); dfree(h->p_batch_off);
+    dfree(h->p_neg_off); dfree(h->p_alpha); dfree(h->det_scratch);
+    dfree(h->counters); dfree(h->queue); dfree(h->loss); dfree(h->g_in); dfree(h->g_out); dfree(h->win_buf);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+    return 0;
+}
+
+const char *par_sgns_last_error(par_sgns_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+int32_t par_sgns_row_stride(par_sgns_handle h) { return h ? h->ld : PAR_SGNS_EINVAL; }
+
+int par_sgns_set_exp_table(par_sgns_handle h, const float *t, int32_t n) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!t || n != sgns::kExpTable + 1) return seterr(h, PAR_SGNS_EINVAL, "exp table must have 1001 entries");
+    int rc = upload(h, h->exp_table, t, (size_t)n);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->have_exp = true;
+    return 0;
+}
+
+int par_sgns_set_sampler(par_sgns_handle h, const int64_t *entries, int32_t V) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!entries || V != h->cfg.vocab_size) return seterr(h, PAR_SGNS_EINVAL, "sampler needs vocab_size entries");
+    // distribution of one negative draw before in-sentence rejection:
+    // slot word i (i >= 1), or a "</s>" slot replaced by uniform [1, V-1]
+    double total = 0;
+    for (int32_t i = 0; i < V; i++) {
+        if (entries[i] < 0) return seterr(h, PAR_SGNS_EINVAL, "negative table entry count");
+        total += (double)entries[i];
+    }
+    if (total <= 0) return seterr(h, PAR_SGNS_EINVAL, "empty unigram table");
+    std::vector<double> w((size_t)V);
+    const double uni = (double)entries[0] / total / (double)(V - 1);
+    w[0] = 0.0;
+    for (int32_t i = 1; i < V; i++) w[(size_t)i] = (double)entries[i] / total + uni;
+    // Vose alias construction
+    std::vector<double> q((size_t)V);
+    std::vector<float> prob((size_t)V, 1.0f);
+    std::vector<int32_t> alias((size_t)V);
+    std::vector<int32_t> small, large;
+    for (int32_t i = 0; i < V; i++) {
+        q[(size_t)i] = w[(size_t)i] * V;
+        alias[(size_t)i] = i;
+        (q[(size_t)i] < 1.0 ? small : large).push_back(i);
+    }
+    while (!small.empty() && !large.empty()) {
+        int32_t s = small.back(), l = large.back();
+        small.pop_back();
+        prob[(size_t)s] = (float)q[(size_t)s];
+        alias[(size_t)s] = l;
+        q[(size_t)l] = (q[(size_t)l] + q[(size_t)s]) - 1.0;
+        if (q[(size_t)l] < 1.0) {
+            large.pop_back();
+            small.push_back(l);
+        }
+    }
+    for (int32_t i : large) { prob[(size_t)i] = 1.0f; alias[(size_t)i] = i; }
+    for (int32_t i : small) { prob[(size_t)i] = 1.0f; alias[(size_t)i] = i; }
+    // a word with zero mass must never be returned by its own bucket
+    if (prob[0] > 0.0f && w[0] == 0.0) prob[0] = 0.0f;
+    if (alias[0] == 0) alias[0] = 1;
+    int rc = upload(h, h->alias_prob, prob.data(), (size_t)V);
+    if (rc) return rc;
+    rc = upload(h, h->alias_idx, alias.data(), (size_t)V);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->have_sampler = true;
+    return 0;
+}
+
+int par_sgns_set_weights(par_sgns_handle h, const float *w_in, const float *w_out) {
+    if (!h) return PAR_SGNS_EINVAL;
+    const int64_t V = h->cfg.vocab_size, d = h->cfg.dim, ld = h->ld;
+    const size_t esz = h->bf16 ? 2 : 4;
+    std::vector<char> buf((size_t)V * ld * esz);
+    const float *srcs[2] = {w_in, w_out};
+    void *dsts[2] = {h->w_in, h->w_out};
+    for (int k = 0; k < 2; k++) {
+        const float *src = srcs[k];
+        if (!src) continue;
+        std::memset(buf.data(), 0, buf.size());
+        if (h->bf16) {
+            uint16_t *o = reinterpret_cast<uint16_t *>(buf.data());
+#pragma omp parallel for
+            for (int64_t r = 0; r < V; r++)
+                for (int64_t j = 0; j < d; j++) o[r * ld + j] = f2bf_rne(src[r * d + j]);
+        } else {
+            float *o = reinterpret_cast<float *>(buf.data());
+#pragma omp parallel for
+            for (int64_t r = 0; r < V; r++) std::memcpy(o + r * ld, src + r * d, (size_t)d * 4);
+        }
+        HIPCHK(h, hipMemcpy(dsts[k], buf.data(), buf.size(), hipMemcpyHostToDevice));
+    }
+    h->have_weights = true;
+    return 0;
+}
+
+int par_sgns_get_weights(par_sgns_handle h, float *w_in, float *w_out) {
+    if (!h) return PAR_SGNS_EINVAL;
+    const int64_t V = h->cfg.vocab_size, d = h->cfg.dim, ld = h->ld;
+    const size_t esz = h->bf16 ? 2 : 4;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    std::vector<char> buf((size_t)V * ld * esz);
+    float *dsts[2] = {w_in, w_out};
+    void *srcs[2] = {h->w_in, h->w_out};
+    for (int k = 0; k < 2; k++) {
+        float *dst = dsts[k];
+        if (!dst) continue;
+        HIPCHK(h, hipMemcpy(buf.data(), srcs[k], buf.size(), hipMemcpyDeviceToHost));
+        if (h->bf16) {
+            const uint16_t *s = reinterpret_cast<const uint16_t *>(buf.data());
+#pragma omp parallel for
+            for (int64_t r = 0; r < V; r++)
+                for (int64_t j = 0; j < d; j++) dst[r * d + j] = bf2f(s[r * ld + j]);
+        } else {
+            const float *s = reinterpret_cast<const float *>(buf.data());
+#pragma omp parallel for
+            for (int64_t r = 0; r < V; r++) std::memcpy(dst + r * d, s + r * ld, (size_t)d * 4);
+        }
+    }
+    return 0;
+}
+
+int par_sgns_set_corpus(par_sgns_handle h, const int32_t *tokens, int64_t n_tokens, const int64_t *sent_off,
+                        const int64_t *sent_raw, int64_t n_sent) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!tokens || !sent_off || !sent_raw || n_tokens < 0 || n_sent < 0)
+        return seterr(h, PAR_SGNS_EINVAL, "null corpus array");
+    int rc = check_offsets(h, sent_off, n_sent, n_tokens, sgns::kMaxSentence, "sent_off");
+    if (rc) return rc;
+    if ((rc = check_ids(h, tokens, n_tokens, 0, h->cfg.vocab_size, "tokens"))) return rc;
+    for (int64_t s = 0; s < n_sent; s++)
+        if (sent_raw[s] < 0) return seterr(h, PAR_SGNS_EINVAL, "sent_raw[%lld] < 0", (long long)s);
+    if ((rc = upload(h, h->tokens, tokens, (size_t)n_tokens))) return rc;
+    if ((rc = upload(h, h->sent_off, sent_off, (size_t)n_sent + 1))) return rc;
+    if ((rc = upload(h, h->sent_raw, sent_raw, (size_t)n_sent))) return rc;
+    std::vector<int32_t> ident((size_t)std::max<int64_t>(n_sent, 1));
+    std::iota(ident.begin(), ident.end(), 0);
+    if ((rc = upload(h, h->order, ident.data(), ident.size()))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->h_sent_off.assign(sent_off, sent_off + n_sent + 1);
+    h->n_tokens = n_tokens;
+    h->n_sent = n_sent;
+    h->ordered.clear();
+    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+    return 0;
+}
+
+int par_sgns_set_plan(par_sgns_handle h, const int32_t *tokens, const uint8_t *b, int64_t n_tokens,
+                      const int64_t *sent_off, const float *sent_alpha, int64_t n_sent, const int64_t *batch_off,
+                      const int64_t *neg_off, int64_t n_batches, const int32_t *negs, int64_t n_negs) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!tokens || !b || !sent_off || !sent_alpha || !batch_off || !neg_off || (!negs && n_negs > 0))
+        return seterr(h, PAR_SGNS_EINVAL, "null plan array");
+    const int W = h->cfg.window, B = h->cfg.batch_size, NEG = h->cfg.negative;
+    int rc = check_offsets(h, sent_off, n_sent, n_tokens, sgns::kMaxSentence, "plan sent_off");
+    if (rc) return rc;
+    if ((rc = check_ids(h, tokens, n_tokens, 0, h->cfg.vocab_size, "plan tokens"))) return rc;
+    if ((rc = check_ids(h, negs, n_negs, 0, h->cfg.vocab_size, "plan negs"))) return rc;
+    for (int64_t i = 0; i < n_tokens; i++)
+        if (b[i] >= W) return seterr(h, PAR_SGNS_EINVAL, "plan b[%lld] >= window", (long long)i);
+    if (batch_off[0] != 0 || neg_off[0] != 0) return seterr(h, PAR_SGNS_EINVAL, "plan offsets must start at 0");
+    for (int64_t s = 0; s < n_sent; s++) {
+        int64_t L = sent_off[s + 1] - sent_off[s];
+        if (batch_off[s + 1] - batch_off[s] != (L + B - 1) / B)
+            return seterr(h, PAR_SGNS_EINVAL, "plan batch_off inconsistent at sentence %lld", (long long)s);
+    }
+    if (batch_off[n_sent] != n_batches) return seterr(h, PAR_SGNS_EINVAL, "plan batch count mismatch");
+    for (int64_t k = 0; k < n_batches; k++) {
+        int64_t K = neg_off[k + 1] - neg_off[k];
+        if (K < 0 || K > (int64_t)(2 * W - 1) * NEG || K % (NEG > 0 ? NEG : 1) != 0)
+            return seterr(h, PAR_SGNS_EINVAL, "plan batch %lld has %lld negatives", (long long)k, (long long)K);
+    }
+    if (neg_off[n_batches] != n_negs) return seterr(h, PAR_SGNS_EINVAL, "plan negative count mismatch");
+    if ((rc = upload(h, h->p_tokens, tokens, (size_t)n_tokens))) return rc;
+    if ((rc = upload(h, h->p_b, b, (size_t)n_tokens))) return rc;
+    if ((rc = upload(h, h->p_sent_off, sent_off, (size_t)n_sent + 1))) return rc;
+    if ((rc = upload(h, h->p_alpha, sent_alpha, (size_t)n_sent * 2))) return rc;
+    if ((rc = upload(h, h->p_batch_off, batch_off, (size_t)n_sent + 1))) return rc;
+    if ((rc = upload(h, h->p_neg_off, neg_off, (size_t)n_batches + 1))) return rc;
+    if ((rc = upload(h, h->p_negs, negs, (size_t)n_negs))) return rc;
+    const size_t ld = (size_t)h->ld, kx = (size_t)h->kmax;
+    if ((rc = dalloc(h, h->det_scratch, (size_t)B * ld * 2 + kx * ld * 2 + kx * (size_t)B))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->p_n_sent = n_sent;
+    return 0;
+}
+
+static int ensure_order(par_sgns_handle h, int64_t b, int64_t e) {
+    if (h->ordered.count({b, e})) return 0;
+    // longest sentences first within the range (balances uneven lengths)
+    std::vector<int32_t> ord((size_t)(e - b));
+    std::iota(ord.begin(), ord.end(), (int32_t)b);
+    const int64_t *off = h->h_sent_off.data();
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+        return off[x + 1] - off[x] > off[y + 1] - off[y];
+    });
+    HIPCHK(h, hipMemcpyAsync(h->order + b, ord.data(), ord.size() * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->ordered.insert({b, e});
+    return 0;
+}
+
+static void default_shape(par_sgns_handle h, int lds_per_wave, int &wpb, int &blocks) {
+    if (wpb <= 0) {
+        wpb = 4;
+        while (wpb > 1 && (size_t)lds_per_wave * wpb > 64 * 1024) wpb /= 2;
+    }
+    if (blocks <= 0) {
+        const int lds_cu = 160 * 1024;
+        int waves_cu = lds_per_wave > 0 ? lds_cu / lds_per_wave : 32;
+        if (waves_cu > 32) waves_cu = 32;
+        int bpc = waves_cu / wpb;
+        if (bpc < 1) bpc = 1;
+        blocks = bpc * (h->cu_count > 0 ? h->cu_count : 256);
+    }
+}
+
+int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_stats *stats) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!a) return seterr(h, PAR_SGNS_EINVAL, "null args");
+    if (!h->have_exp || !h->have_weights) return seterr(h, PAR_SGNS_ESTATE, "exp table and weights must be set first");
+    if (a->iter < 1 || a->epoch < 0 || a->sent_begin < 0 || a->sent_end < a->sent_begin)
+        return seterr(h, PAR_SGNS_EINVAL, "bad schedule arguments");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc;
+    float ms = 0.f;
+    int launches = 1;
+    if (a->mode == PAR_SGNS_DETERMINISTIC) {
+        if (h->bf16) return seterr(h, PAR_SGNS_EINVAL, "deterministic mode runs on fp32 tables");
+        if (!h->p_tokens) return seterr(h, PAR_SGNS_ESTATE, "no plan uploaded");
+        if (a->sent_end > h->p_n_sent) return seterr(h, PAR_SGNS_EINVAL, "sentence range beyond the plan");
+        sgns::DetArgs d{};
+        d.tokens = h->p_tokens; d.b = h->p_b; d.sent_off = h->p_sent_off; d.sent_alpha = h->p_alpha;
+        d.batch_off = h->p_batch_off; d.neg_off = h->p_neg_off; d.negs = h->p_negs; d.exp_table = h->exp_table;
+        d.w_in = (float *)h->w_in; d.w_out = (float *)h->w_out;
+        const size_t ld = (size_t)h->ld, B = (size_t)h->cfg.batch_size, kx = (size_t)h->kmax;
+        d.m_in = h->det_scratch;
+        d.m_in_new = d.m_in + B * ld;
+        d.m_out = d.m_in_new + B * ld;
+        d.m_out_upd = d.m_out + kx * ld;
+        d.m_grad = d.m_out_upd + kx * ld;
+        d.ld = h->ld; d.d = h->cfg.dim; d.W = h->cfg.window; d.B = h->cfg.batch_size; d.kmax = h->kmax;
+        d.compute_loss = a->compute_loss; d.s_begin = a->sent_begin; d.s_end = a->sent_end;
+        d.loss = h->loss; d.counters = h->counters;
+        HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, nullptr, h->st));
+        HIPCHK(h, hipEventRecord(h->ev0, h->st));
+        HIPCHK(h, sgns::launch_det(d, h->st));
+        HIPCHK(h, hipEventRecord(h->ev1, h->st));
+    } else if (a->mode == PAR_SGNS_PLAIN || a->mode == PAR_SGNS_PERSISTENT) {
+        if (!h->have_sampler) return seterr(h, PAR_SGNS_ESTATE, "sampler not set");
+        if (!h->tokens) return seterr(h, PAR_SGNS_ESTATE, "no corpus uploaded");
+        if (a->sent_end > h->n_sent) return seterr(h, PAR_SGNS_EINVAL, "sentence range beyond the corpus");
+        const bool pers = a->mode == PAR_SGNS_PERSISTENT;
+        sgns::FastArgs f{};
+        f.tokens = h->tokens; f.sent_off = h->sent_off; f.sent_raw = h->sent_raw; f.order = pers ? h->order : nullptr;
+        f.exp_table = h->exp_table; f.alias_prob = h->alias_prob; f.alias_idx = h->alias_idx;
+        f.w_in = h->w_in; f.w_out = h->w_out;
+        f.ld = h->ld; f.d = h->cfg.dim; f.nq = h->nq; f.W = h->cfg.window; f.R = h->R; f.NEG = h->cfg.negative;
+        f.B = h->cfg.batch_size; f.V = h->cfg.vocab_size; f.kmax = h->kmax;
+        f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax);
+        f.persistent = pers ? 1 : 0; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
+        f.a0 = a->alpha; f.T = a->train_words; f.seed = a->seed; f.s_begin = a->sent_begin; f.s_end = a->sent_end;
+        f.queue = h->queue; f.loss = h->loss; f.counters = h->counters;
+        int wpb = pers ? a->waves_per_block : 1, blocks = a->blocks;
+        default_shape(h, f.lds_per_wave, wpb, blocks);
+        if (!pers) wpb = a->waves_per_block > 0 ? a->waves_per_block : 1;
+        if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
+            return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
+        if (pers && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
+        if (pers && a->use_graph) {
+            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss, wpb, blocks, a->iter, 0, a->seed, a->train_words};
+            std::memcpy(&k.alpha_bits, &a->alpha, 4);
+            auto it = h->graphs.find(k);
+            if (it == h->graphs.end()) {
+                hipGraph_t g;
+                HIPCHK(h, hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+                hipError_t e1 = sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, h->queue, h->st);
+                hipError_t e2 = e1 == hipSuccess ? sgns::launch_fast(f, h->bf16, h->np, wpb, blocks, h->st) : e1;
+                hipError_t e3 = hipStreamEndCapture(h->st, &g);
+                if (e2 != hipSuccess || e3 != hipSuccess)
+                    return seterr(h, PAR_SGNS_EDEVICE, "graph capture failed: %s",
+                                  hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+                hipGraphExec_t ge;
+                hipError_t e4 = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e4 != hipSuccess) return seterr(h, PAR_SGNS_EDEVICE, "graph instantiate: %s", hipGetErrorString(e4));
+                it = h->graphs.emplace(k, ge).first;
+            }
+            HIPCHK(h, hipEventRecord(h->ev0, h->st));
+            HIPCHK(h, hipGraphLaunch(it->second, h->st));
+            HIPCHK(h, hipEventRecord(h->ev1, h->st));
+        } else {
+            HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, h->queue, h->st));
+            HIPCHK(h, hipEventRecord(h->ev0, h->st));
+            HIPCHK(h, sgns::launch_fast(f, h->bf16, h->np, wpb, blocks, h->st));
+            HIPCHK(h, hipEventRecord(h->ev1, h->st));
+        }
+    } else {
+        return seterr(h, PAR_SGNS_EINVAL, "unknown mode %d", a->mode);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms[0] = ms;
+    h->last_launches[0] = launches;
+    if (stats) {
+        unsigned long long c[sgns::kCntSlots];
+        double l[2];
+        HIPCHK(h, hipMemcpy(c, h->counters, sizeof c, hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(l, h->loss, sizeof l, hipMemcpyDeviceToHost));
+        stats->loss_attr = l[0];
+        stats->loss_rep = l[1];
+        stats->tokens = c[sgns::kCntTokens];
+        stats->sentences = c[sgns::kCntSentences];
+        stats->pairs = c[sgns::kCntPairs];
+        stats->batches = c[sgns::kCntBatches];
+        stats->negatives = c[sgns::kCntNegatives];
+        stats->rep_pairs = c[sgns::kCntRepPairs];
+        stats->kernel_ms = ms;
+    }
+    return 0;
+}
+
+static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg, int64_t nw,
+                       int32_t nctx, int32_t nneg, int32_t flags, double *loss_out, int adjoint) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!centre || !ctx || (nneg > 0 && !neg) || nw < 0 || nctx < 1 || nctx > 32 || nneg < 0 || nneg > 7)
+        return seterr(h, PAR_SGNS_EINVAL, "bad window batch (nctx 1..32, nneg 0..7)");
+    if (!h->have_weights) return seterr(h, PAR_SGNS_ESTATE, "weights not set");
+    if ((flags & PAR_SGNS_WIN_MFMA) && !h->bf16) return seterr(h, PAR_SGNS_EINVAL, "MFMA path needs bf16 tables");
+    const int32_t V = h->cfg.vocab_size;
+    int rc;
+    if ((rc = check_ids(h, centre, nw, 0, V, "centre"))) return rc;
+    if ((rc = check_ids(h, ctx, nw * nctx, 0, V, "ctx"))) return rc;
+    if ((rc = check_ids(h, neg, nw * nneg, 0, V, "neg"))) return rc;
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    const size_t need = (size_t)nw * (size_t)(1 + nctx + nneg);
+    if (need > h->win_cap) {
+        if ((rc = dalloc(h, h->win_buf, need))) return rc;
+        h->win_cap = need;
+    }
+    int32_t *dc = h->win_buf, *dx = dc + nw, *dn = dx + nw * nctx;
+    HIPCHK(h, hipMemcpyAsync(dc, centre, (size_t)nw * 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(dx, ctx, (size_t)nw * nctx * 4, hipMemcpyHostToDevice, h->st));
+    if (nneg > 0) HIPCHK(h, hipMemcpyAsync(dn, neg, (size_t)nw * nneg * 4, hipMemcpyHostToDevice, h->st));
+    const size_t rows = (size_t)V * (size_t)h->ld;
+    if (adjoint) {
+        if (!h->g_in) {
+            if ((rc = dalloc(h, h->g_in, rows))) return rc;
+            if ((rc = dalloc(h, h->g_out, rows))) return rc;
+        }
+        HIPCHK(h, hipMemsetAsync(h->g_in, 0, rows * 4, h->st));
+        HIPCHK(h, hipMemsetAsync(h->g_out, 0, rows * 4, h->st));
+    }
+    HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, nullptr, h->st));
+    sgns::WindowArgs w{};
+    w.centre = dc; w.ctx = dx; w.neg = dn; w.n_windows = nw; w.nctx = nctx; w.nneg = nneg;
+    w.d = h->cfg.dim; w.ld = h->ld; w.w_in = h->w_in; w.w_out = h->w_out; w.g_in = h->g_in; w.g_out = h->g_out;
+    w.loss = h->loss;
+    HIPCHK(h, hipEventRecord(h->ev0, h->st));
+    hipError_t e = sgns::launch_window(w, h->bf16, h->np, adjoint, flags, h->st);
+    if (e == hipErrorNotSupported) return seterr(h, PAR_SGNS_EINVAL, "requested window path is not built");
+    HIPCHK(h, e);
+    HIPCHK(h, hipEventRecord(h->ev1, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms[adjoint ? 2 : 1] = ms;
+    h->last_launches[adjoint ? 2 : 1] = 1;
+    double l[2];
+    HIPCHK(h, hipMemcpy(l, h->loss, sizeof l, hipMemcpyDeviceToHost));
+    if (loss_out) *loss_out = l[0];
+    return 0;
+}
+
+int par_sgns_loss_forward(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
+                          int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out) {
+    return window_call(h, centre, ctx, neg, n_windows, nctx, nneg, flags, loss_out, 0);
+}
+
+int par_sgns_loss_adjoint(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
+                          int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out) {
+    return window_call(h, centre, ctx, neg, n_windows, nctx, nneg, flags, loss_out, 1);
+}
+
+int par_sgns_get_grads(par_sgns_handle h, float *g_in, float *g_out) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!h->g_in) return seterr(h, PAR_SGNS_ESTATE, "no adjoint has run");
+    const int64_t V = h->cfg.vocab_size, d = h->cfg.dim, ld = h->ld;
+    std::vector<float> buf((size_t)V * ld);
+    float *dsts[2] = {g_in, g_out};
+    float *srcs[2] = {h->g_in, h->g_out};
+    for (int k = 0; k < 2; k++) {
+        if (!dsts[k]) continue;
+        HIPCHK(h, hipMemcpy(buf.data(), srcs[k], buf.size() * 4, hipMemcpyDeviceToHost));
+        for (int64_t r = 0; r < V; r++) std::memcpy(dsts[k] + r * d, buf.data() + r * ld, (size_t)d * 4);
+    }
+    return 0;
+}
+
+int par_sgns_last_kernel_time(par_sgns_handle h, int32_t kind, double *ms, int32_t *launches) {
+    if (!h || kind < 0 || kind > 2) return PAR_SGNS_EINVAL;
+    if (ms) *ms = h->last_ms[kind];
+    if (launches) *launches = h->last_launches[kind];
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// sgns_device.h -- gfx950 device helpers shared by the PAR-SGNS kernels.
+//
+// Row layout on the device: W_in / W_out are [V][ld] in float or bf16, ld a
+// multiple of 32 (fp32) or 64 (bf16) so every row starts on a 128-byte line.
+// A wavefront covers a row in element PAIRS: lane l owns pairs l, l+64, ...
+// (NP pairs per lane), so each load/store instruction of the wave touches
+// 64 consecutive pairs (512 B fp32, 256 B bf16) and LDS copies are
+// bank-conflict free.  Pairs at or beyond nq = ceil(d/2) are idle; padding
+// elements are zero and stay zero under every update (g*0 = 0).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgns {
+
+constexpr int kWave = 64;
+constexpr int kExpTable = 1000;
+constexpr int kMaxExp = 6;
+constexpr int kMaxSentence = 1000;
+
+// ------------------------------------------------------------- reductions
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// Sum over the 64 lanes, result uniform.  Butterfly inside each row of 16
+// (quad_perm xor1, xor2, row_half_mirror, row_mirror), then the four row sums.
+__device__ __forceinline__ float wave_sum(float v) {
+    v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp<0x141>(v);  // row_half_mirror
+    v += dpp<0x140>(v);  // row_mirror
+    return (readlane_f(v, 0) + readlane_f(v, 16)) + (readlane_f(v, 32) + readlane_f(v, 48));
+}
+
+__device__ __forceinline__ int64_t readfirst_i64(int64_t v) {
+    int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)v);
+    int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// --------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with the
+// published multipliers and Weyl key increments.
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+// Counter layout of one random draw: (index, purpose tag, sentence lo,
+// sentence hi ^ epoch<<20).  Every draw of the trainer is a pure function of
+// the key, the sentence, the epoch and its index, independent of scheduling.
+enum : uint32_t { kTagWindow = 1, kTagRepWindow = 2, kTagNeg = 3, kTagRedraw = 4, kTagRound = 5 };
+
+__device__ __forceinline__ U4 draw(uint64_t seed, int64_t sent, uint32_t epoch, uint32_t tag, uint32_t idx) {
+    return philox4x32_10(U4{idx, tag, (uint32_t)(uint64_t)sent, (uint32_t)((uint64_t)sent >> 32) ^ (epoch << 20)},
+                         (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// 32-bit integer hash (lowbias32 by C. Wellons) for rounding noise.
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// ---------------------------------------------------------------- bf16
+__device__ __forceinline__ float bf16lo(uint32_t p) { return __builtin_bit_cast(float, p << 16); }
+__device__ __forceinline__ float bf16hi(uint32_t p) { return __builtin_bit_cast(float, p & 0xFFFF0000u); }
+
+// Stochastic rounding to bf16: add 16 uniform bits below the kept mantissa
+// and truncate.  Rounds up with probability equal to the discarded fraction.
+__device__ __forceinline__ uint32_t bf16_sr(float x, uint32_t noise16) {
+    uint32_t b = __builtin_bit_cast(uint32_t, x);
+    return (b + (noise16 & 0xFFFFu)) >> 16;
+}
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+    uint32_t b = __builtin_bit_cast(uint32_t, x);
+    return (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+}
+
+// ------------------------------------------------------------- row pairs
+// Storage traits: a "pair" is two adjacent elements of a row.
+template <typename T> struct Pair;
+template <> struct Pair<float> {
+    using P = float2;
+    __device__ static __forceinline__ float2 load(const float *w, int64_t row, int ld, int q) {
+        return reinterpret_cast<const float2 *>(w + row * ld)[q];
+    }
+    __device__ static __forceinline__ float2 tof(float2 p) { return p; }
+};
+template <> struct Pair<uint16_t> {  // bf16 bit patterns
+    using P = uint32_t;
+    __device__ static __forceinline__ uint32_t load(const uint16_t *w, int64_t row, int ld, int q) {
+        return reinterpret_cast<const uint32_t *>(w + row * ld)[q];
+    }
+    __device__ static __forceinline__ float2 tof(uint32_t p) { return make_float2(bf16lo(p), bf16hi(p)); }
+};
+
+// sigmoid-table gradient of a pair, cpu:554-559 (label 1) and cpu:642-647
+// (label 0); table index (int)((f + 6) * 83.333336f).
+__device__ __forceinline__ float grad_label1(float f, float alpha, const float *et) {
+    if (f > kMaxExp) return 0.0f;
+    if (f < -kMaxExp) return alpha;
+    return (1.0f - et[(int)((f + (float)kMaxExp) * (1000.0f / 12.0f))]) * alpha;
+}
+__device__ __forceinline__ float grad_label0(float f, float alpha, const float *et) {
+    if (f > kMaxExp) return -alpha;
+    if (f < -kMaxExp) return 0.0f;
+    return (0.0f - et[(int)((f + (float)kMaxExp) * (1000.0f / 12.0f))]) * alpha;
+}
+
+__device__ __forceinline__ float softplusf(float x) {  // log(1 + e^x)
+    return x > 0.0f ? x + log1pf(__expf(-x)) : log1pf(__expf(x));
+}
+
+// Learning rate of a sentence that starts `raw` words into the epoch: the
+// schedule of cpu:443-445 evaluated at the sentence start.
+__device__ __forceinline__ float sentence_alpha(float a0, int32_t epoch, uint64_t T, int32_t iter, int64_t raw) {
+    float wca = (float)((uint64_t)epoch * T + (uint64_t)raw);
+    float a = a0 * (1.0f - wca / (float)((uint64_t)iter * T + 1));
+    float fl = a0 * 0.0001f;
+    return a < fl ? fl : a;
+}
+
+}  // namespace sgns

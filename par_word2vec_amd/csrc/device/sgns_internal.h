@@ -1,0 +1,73 @@
+// sgns_internal.h -- argument blocks and launchers shared between the kernel
+// translation units and the C-ABI layer (not part of the public interface).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgns {
+
+// Device counters, one slot each (uint64), reset before every training call.
+enum Counter { kCntTokens = 0, kCntSentences, kCntPairs, kCntBatches, kCntNegatives, kCntRepPairs, kCntSlots = 8 };
+
+struct DetArgs {
+    const int32_t *tokens;
+    const uint8_t *b;
+    const int64_t *sent_off;
+    const float *sent_alpha;
+    const int64_t *batch_off;
+    const int64_t *neg_off;
+    const int32_t *negs;
+    const float *exp_table;
+    float *w_in, *w_out;
+    float *m_in, *m_out, *m_grad, *m_out_upd, *m_in_new;  // scratch
+    int32_t ld, d, W, B, kmax;
+    int32_t compute_loss;
+    int64_t s_begin, s_end;
+    double *loss;                 // [2]
+    unsigned long long *counters; // [kCntSlots]
+};
+
+struct FastArgs {
+    const int32_t *tokens;
+    const int64_t *sent_off;
+    const int64_t *sent_raw;
+    const int32_t *order;         // persistent queue order (may be null)
+    const float *exp_table;
+    const float *alias_prob;
+    const int32_t *alias_idx;
+    void *w_in, *w_out;
+    int32_t ld, d, nq, W, R, NEG, B, V, kmax;
+    int32_t lds_per_wave;         // bytes
+    int32_t persistent;
+    int32_t compute_loss;
+    int32_t iter, epoch;
+    float a0;
+    uint64_t T, seed;
+    int64_t s_begin, s_end;
+    unsigned long long *queue;
+    double *loss;
+    unsigned long long *counters;
+};
+
+struct WindowArgs {
+    const int32_t *centre, *ctx, *neg;
+    int64_t n_windows;
+    int32_t nctx, nneg, d, ld;
+    const void *w_in, *w_out;
+    float *g_in, *g_out;          // adjoint outputs (dense, [V][ld])
+    double *loss;
+};
+
+hipError_t launch_det(const DetArgs &a, hipStream_t st);
+hipError_t launch_fast(const FastArgs &a, int bf16, int np, int waves_per_block, int blocks, hipStream_t st);
+hipError_t launch_reset(unsigned long long *counters, int n_counters, double *loss, unsigned long long *queue,
+                        hipStream_t st);
+hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int flags, hipStream_t st);
+
+// per-wave LDS bytes of the fast kernel
+inline int fast_lds_per_wave(int R, int nq, int bf16, int kmax) {
+    int ring = R * nq * (8 + (bf16 ? 4 : 8));
+    return ((ring + kmax * 4) + 15) & ~15;
+}
+
+}  // namespace sgns
