@@ -35,6 +35,9 @@ enum {
 };
 
 enum par_sgns_precision { PAR_SGNS_FP32 = 0, PAR_SGNS_BF16 = 1 };
+/* How a bf16 table entry absorbs an fp32 update: stochastic rounding (the
+ * default; unbiased for updates below half an ulp) or round-to-nearest. */
+enum par_sgns_rounding { PAR_SGNS_ROUND_STOCHASTIC = 0, PAR_SGNS_ROUND_NEAREST = 1 };
 
 enum par_sgns_mode {
     PAR_SGNS_DETERMINISTIC = 0, /* one wavefront replays an uploaded plan exactly */
@@ -52,7 +55,7 @@ typedef struct par_sgns_config {
     int32_t batch_size;   /* -batch-size (repulsion batch B), 1..64 */
     int32_t precision;    /* par_sgns_precision */
     int32_t device;       /* HIP device ordinal */
-    int32_t reserved;
+    int32_t rounding;     /* bf16 write-back: par_sgns_rounding */
 } par_sgns_config;
 
 typedef struct par_sgns_train_args {
@@ -64,6 +67,9 @@ typedef struct par_sgns_train_args {
     int32_t waves_per_block; /* persistent mode; 0 = tuned default */
     int32_t blocks;          /* persistent mode; 0 = tuned default */
     int32_t use_graph;       /* persistent mode: replay a captured hipGraph */
+    int32_t schedule;        /* plain/persistent: 0 = Philox draws over the uploaded
+                                corpus, 1 = the uploaded exact plan (reference draws) */
+    int32_t reserved;
     uint64_t train_words; /* T = sum of vocabulary counts */
     uint64_t seed;        /* Philox key for windows, negatives, rounding */
     int64_t sent_begin;   /* sentence range [begin, end) of the uploaded corpus */

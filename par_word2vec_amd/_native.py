@@ -31,12 +31,13 @@ def _load(name: str) -> C.CDLL:
 # ----------------------------------------------------------------- structs
 class ParSgnsConfig(C.Structure):
     _fields_ = [("vocab_size", i32), ("dim", i32), ("window", i32), ("negative", i32),
-                ("batch_size", i32), ("precision", i32), ("device", i32), ("reserved", i32)]
+                ("batch_size", i32), ("precision", i32), ("device", i32), ("rounding", i32)]
 
 
 class ParSgnsTrainArgs(C.Structure):
     _fields_ = [("mode", i32), ("iter", i32), ("epoch", i32), ("compute_loss", i32),
                 ("alpha", f32), ("waves_per_block", i32), ("blocks", i32), ("use_graph", i32),
+                ("schedule", i32), ("reserved", i32),
                 ("train_words", u64), ("seed", u64), ("sent_begin", i64), ("sent_end", i64)]
 
 

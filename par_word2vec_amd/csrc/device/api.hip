@@ -163,7 +163,8 @@ int par_sgns_create(const par_sgns_config *cfg, par_sgns_handle *out) {
     *out = nullptr;
     if (cfg->vocab_size < 2 || cfg->dim < 1 || cfg->dim > 512 || cfg->window < 1 || cfg->window > 64 ||
         cfg->negative < 0 || cfg->negative > 64 || cfg->batch_size < 1 || cfg->batch_size > 64 ||
-        (cfg->precision != PAR_SGNS_FP32 && cfg->precision != PAR_SGNS_BF16))
+        (cfg->precision != PAR_SGNS_FP32 && cfg->precision != PAR_SGNS_BF16) ||
+        (cfg->rounding != PAR_SGNS_ROUND_STOCHASTIC && cfg->rounding != PAR_SGNS_ROUND_NEAREST))
         return PAR_SGNS_EINVAL;
     par_sgns_handle h = new par_sgns_handle_s;
     h->cfg = *cfg;
@@ -457,18 +458,31 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
         HIPCHK(h, sgns::launch_det(d, h->st));
         HIPCHK(h, hipEventRecord(h->ev1, h->st));
     } else if (a->mode == PAR_SGNS_PLAIN || a->mode == PAR_SGNS_PERSISTENT) {
-        if (!h->have_sampler) return seterr(h, PAR_SGNS_ESTATE, "sampler not set");
-        if (!h->tokens) return seterr(h, PAR_SGNS_ESTATE, "no corpus uploaded");
-        if (a->sent_end > h->n_sent) return seterr(h, PAR_SGNS_EINVAL, "sentence range beyond the corpus");
+        const bool planned = a->schedule == 1;
+        if (planned) {
+            if (!h->p_tokens) return seterr(h, PAR_SGNS_ESTATE, "no plan uploaded");
+            if (a->sent_end > h->p_n_sent) return seterr(h, PAR_SGNS_EINVAL, "sentence range beyond the plan");
+        } else {
+            if (!h->have_sampler) return seterr(h, PAR_SGNS_ESTATE, "sampler not set");
+            if (!h->tokens) return seterr(h, PAR_SGNS_ESTATE, "no corpus uploaded");
+            if (a->sent_end > h->n_sent) return seterr(h, PAR_SGNS_EINVAL, "sentence range beyond the corpus");
+        }
         const bool pers = a->mode == PAR_SGNS_PERSISTENT;
         sgns::FastArgs f{};
         f.tokens = h->tokens; f.sent_off = h->sent_off; f.sent_raw = h->sent_raw; f.order = pers ? h->order : nullptr;
+        if (planned) {
+            // exact schedule: plan tokens in plan order (sentence order is kept so a
+            // single wave replays the reference's sequence)
+            f.tokens = h->p_tokens; f.sent_off = h->p_sent_off; f.sent_raw = nullptr; f.order = nullptr;
+            f.plan_b = h->p_b; f.plan_alpha = h->p_alpha; f.plan_batch_off = h->p_batch_off;
+            f.plan_neg_off = h->p_neg_off; f.plan_negs = h->p_negs;
+        }
         f.exp_table = h->exp_table; f.alias_prob = h->alias_prob; f.alias_idx = h->alias_idx;
         f.w_in = h->w_in; f.w_out = h->w_out;
         f.ld = h->ld; f.d = h->cfg.dim; f.nq = h->nq; f.W = h->cfg.window; f.R = h->R; f.NEG = h->cfg.negative;
         f.B = h->cfg.batch_size; f.V = h->cfg.vocab_size; f.kmax = h->kmax;
-        f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax);
-        f.persistent = pers ? 1 : 0; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
+        f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size);
+        f.persistent = pers ? 1 : 0; f.rounding = h->cfg.rounding; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
         f.a0 = a->alpha; f.T = a->train_words; f.seed = a->seed; f.s_begin = a->sent_begin; f.s_end = a->sent_end;
         f.queue = h->queue; f.loss = h->loss; f.counters = h->counters;
         int wpb = pers ? a->waves_per_block : 1, blocks = a->blocks;
@@ -476,9 +490,10 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
         if (!pers) wpb = a->waves_per_block > 0 ? a->waves_per_block : 1;
         if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
-        if (pers && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
+        if (pers && !planned && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
         if (pers && a->use_graph) {
-            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss, wpb, blocks, a->iter, 0, a->seed, a->train_words};
+            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0), wpb, blocks, a->iter, 0,
+                       a->seed, a->train_words};
             std::memcpy(&k.alpha_bits, &a->alpha, 4);
             auto it = h->graphs.find(k);
             if (it == h->graphs.end()) {

@@ -6,12 +6,13 @@
 // is a wave so every reduction is a DPP butterfly with no barrier).
 //
 // Attraction (cpu:529-568): the W_in rows of the sliding window [p-W, p+W]
-// live in an LDS ring of R = 2W+1 slots holding the current fp32 value and
-// the value that was loaded; a row is fetched once when it enters the window
-// (prefetched one centre ahead) and written back once when it leaves, as
-// delta = current - loaded.  The centre's W_out row stays in registers for
-// all its pairs.  Pairs of one centre are applied sequentially, exactly as
-// the reference does, so later pairs see earlier updates.
+// live in an LDS ring of R = 2W+1 word-keyed slots holding the current fp32
+// value and the value that was loaded; a row is fetched once when its word
+// enters the window (prefetched one centre ahead) and written back once when
+// its last occurrence leaves, as delta = current - loaded.  The centre's
+// W_out row stays in registers for all its pairs (and across a repeated
+// centre).  Pairs are applied sequentially, exactly as the reference does, so
+// later pairs see earlier updates.
 //
 // Repulsion (cpu:571-679): per batch of B positions, K = rep_b * negative
 // shared negatives are drawn from the on-device alias table with the
@@ -33,19 +34,22 @@ namespace sgns {
 namespace {
 
 constexpr int kMaxRedraw = 64;
+constexpr int kUnusedSlot = -(1 << 30);
 
 // ---------------------------------------------------------------- stores
-__device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, float2 dlt, uint32_t) {
+__device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, float2 dlt, uint32_t, int) {
     float *p = w + row * ld + 2 * q;
     if (dlt.x != 0.0f) unsafeAtomicAdd(p, dlt.x);
     if (dlt.y != 0.0f) unsafeAtomicAdd(p + 1, dlt.y);
 }
-__device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 dlt, uint32_t noise) {
+__device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 dlt, uint32_t noise,
+                                        int nearest) {
     if (dlt.x == 0.0f && dlt.y == 0.0f) return;
     uint32_t *p = reinterpret_cast<uint32_t *>(w + row * ld) + q;
     const uint32_t cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const float x = bf16lo(cur) + dlt.x, y = bf16hi(cur) + dlt.y;
-    const uint32_t out = bf16_sr(x, noise) | (bf16_sr(y, noise >> 16) << 16);
+    const uint32_t out = nearest ? (bf16_rne(x) | (bf16_rne(y) << 16))
+                                 : (bf16_sr(x, noise) | (bf16_sr(y, noise >> 16) << 16));
     __hip_atomic_store(p, out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -72,11 +76,12 @@ __device__ __forceinline__ uint32_t noise_of(uint32_t salt, int64_t row, int q) 
 
 template <typename T, int NP>
 __device__ __forceinline__ void store_delta(T *w, int64_t row, int ld, int nq, int lane, const float2 (&cur)[NP],
-                                            const float2 (&ref)[NP], uint32_t salt) {
+                                            const float2 (&ref)[NP], uint32_t salt, int nearest) {
 #pragma unroll
     for (int t = 0; t < NP; t++) {
         const int q = lane + kWave * t;
-        if (q < nq) wb_pair(w, row, ld, q, make_float2(cur[t].x - ref[t].x, cur[t].y - ref[t].y), noise_of(salt, row, q));
+        if (q < nq)
+            wb_pair(w, row, ld, q, make_float2(cur[t].x - ref[t].x, cur[t].y - ref[t].y), noise_of(salt, row, q), nearest);
     }
 }
 
@@ -92,18 +97,46 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
     const int L = (int)(a.sent_off[s + 1] - o0);
     if (L <= 0) return;  // an empty sentence trains nothing (cpu:539-542, 575)
     const int32_t *sen = a.tokens + o0;
-    const float alpha = sentence_alpha(a.a0, a.epoch, a.T, a.iter, a.sent_raw[s]);
+    const bool planned = a.plan_b != nullptr;
+    const float alpha = planned ? a.plan_alpha[2 * s + 1] : sentence_alpha(a.a0, a.epoch, a.T, a.iter, a.sent_raw[s]);
+    const float alpha0 = planned ? a.plan_alpha[2 * s] : alpha;  // centre 0 (cpu:431-447 timing)
     const uint32_t ep = (uint32_t)a.epoch;
     const uint32_t salt0 = hash32((uint32_t)a.seed ^ hash32((uint32_t)s * 0x632BE5ABu ^ (ep << 24)));
+    // LDS of this wave: ring values (fp32), ring snapshots (storage type),
+    // shared negatives, and the slot tables of the ring
     float2 *ring_u = reinterpret_cast<float2 *>(lds);
     PT *ring_0 = reinterpret_cast<PT *>(lds + (size_t)R * nq * sizeof(float2));
-    int32_t *negs = reinterpret_cast<int32_t *>(lds + (size_t)R * nq * (sizeof(float2) + sizeof(PT)));
+    const int ring_bytes = R * nq * (int)(sizeof(float2) + sizeof(PT));
+    const int snap_bytes = a.B * nq * (int)sizeof(PT);
+    int32_t *negs = reinterpret_cast<int32_t *>(lds + (((ring_bytes > snap_bytes ? ring_bytes : snap_bytes) + 15) & ~15));
+    int32_t *slot_word = negs + a.kmax;  // word held by a slot
+    int32_t *slot_last = slot_word + R;  // last window position referencing it
+    int32_t *pos_slot = slot_last + R;   // slot of position c, indexed c % R
 
     cnt[kCntSentences]++;
     cnt[kCntTokens] += (uint64_t)L;
 
-    auto install = [&](int c, const PT (&row)[NP]) {
-        const int slot = c % R;
+    // Slot of word w among slots still referenced at or after position `from`
+    // (-1 if none).  One lane per slot, first match wins.
+    auto find_slot = [&](int w, int from) -> int {
+        for (int r0 = 0; r0 < R; r0 += kWave) {
+            const int sx = r0 + lane;
+            const bool m = sx < R && slot_last[sx] >= from && slot_word[sx] == w;
+            const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
+            if (mask) return r0 + (int)__builtin_ctzll(mask);
+        }
+        return -1;
+    };
+    // A slot referenced by no position at or after `from`.
+    auto find_free = [&](int from) -> int {
+        for (int r0 = 0; r0 < R; r0 += kWave) {
+            const int sx = r0 + lane;
+            const uint64_t mask = __builtin_amdgcn_ballot_w64(sx < R && slot_last[sx] < from);
+            if (mask) return r0 + (int)__builtin_ctzll(mask);
+        }
+        return 0;  // unreachable: the window references at most R slots
+    };
+    auto fill = [&](int slot, const PT (&row)[NP]) {
 #pragma unroll
         for (int t = 0; t < NP; t++) {
             const int q = lane + kWave * t;
@@ -113,50 +146,79 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
             }
         }
     };
-    auto flush = [&](int c) {
-        const int slot = c % R;
-        const int64_t row = sen[c];
-        const uint32_t salt = salt0 ^ hash32((uint32_t)c + 0x1000u);
+    auto flush = [&](int slot) {
+        const int64_t row = slot_word[slot];
+        const uint32_t salt = salt0 ^ hash32((uint32_t)slot_last[slot] + 0x1000u);
 #pragma unroll
         for (int t = 0; t < NP; t++) {
             const int q = lane + kWave * t;
             if (q < nq) {
                 const float2 u = ring_u[slot * nq + q];
                 const float2 u0 = Pair<T>::tof(ring_0[slot * nq + q]);
-                wb_pair(w_in, row, ld, q, make_float2(u.x - u0.x, u.y - u0.y), noise_of(salt, row, q));
+                wb_pair(w_in, row, ld, q, make_float2(u.x - u0.x, u.y - u0.y), noise_of(salt, row, q), a.rounding);
             }
+        }
+    };
+    auto bind = [&](int c, int slot, int w) {
+        if (lane == 0) {
+            slot_word[slot] = w;
+            slot_last[slot] = c;
+            pos_slot[c % R] = slot;
         }
     };
 
     // ---- attraction
+    // The W_in rows of window [p-W, p+W] live in LDS slots keyed by word: a
+    // word occurring twice in the window shares one slot, so every pair sees
+    // the updates of the pairs before it exactly as cpu:551-565 does in place.
+    // A slot is written back when the last position referencing it leaves the
+    // window; the next centre's W_out row and entering W_in row are prefetched
+    // one centre ahead.
     {
+        // unused slots sit below every window start (window starts go down to -W)
+        for (int r0 = 0; r0 < R; r0 += kWave)
+            if (r0 + lane < R) slot_last[r0 + lane] = kUnusedSlot;
         const int first = W < L - 1 ? W : L - 1;
         for (int c = 0; c <= first; c++) {
-            PT row[NP];
-            load_row<T, NP>(w_in, sen[c], ld, nq, lane, row);
-            install(c, row);
+            const int w = sen[c];
+            int sl = find_slot(w, 0);
+            if (sl < 0) {
+                sl = find_free(0);
+                PT row[NP];
+                load_row<T, NP>(w_in, w, ld, nq, lane, row);
+                fill(sl, row);
+            }
+            bind(c, sl, w);
         }
         PT vnext[NP];
         load_row<T, NP>(w_out, sen[0], ld, nq, lane, vnext);
+        float2 v[NP], v0[NP];
+        bool carried = false;
         uint32_t bwin = 0;
         for (int p = 0; p < L; p++) {
-            if ((p & (kWave - 1)) == 0) bwin = draw(a.seed, s, ep, kTagWindow, (uint32_t)(p + lane)).x % (uint32_t)W;
+            if ((p & (kWave - 1)) == 0)
+                bwin = planned ? (p + lane < L ? a.plan_b[o0 + p + lane] : 0u)
+                               : draw(a.seed, s, ep, kTagWindow, (uint32_t)(p + lane)).x % (uint32_t)W;
             const int b = __builtin_amdgcn_readlane((int)bwin, p & (kWave - 1));
-            float2 v[NP], v0[NP];
-            to_f<T, NP>(vnext, v);
+            const float alpha_p = p == 0 ? alpha0 : alpha;
+            if (!carried) {  // a centre repeated at p-1 keeps its register row
+                to_f<T, NP>(vnext, v);
 #pragma unroll
-            for (int t = 0; t < NP; t++) v0[t] = v[t];
+                for (int t = 0; t < NP; t++) v0[t] = v[t];
+            }
+            const bool carry_next = p + 1 < L && sen[p + 1] == sen[p];
             // prefetch for centre p+1: its W_out row and the W_in row entering its window
             const int np = p + 1 + W;
+            const int np_slot = np < L ? find_slot(sen[np], p - W) : -1;
             PT pf[NP];
-            if (p + 1 < L) load_row<T, NP>(w_out, sen[p + 1], ld, nq, lane, vnext);
-            if (np < L) load_row<T, NP>(w_in, sen[np], ld, nq, lane, pf);
+            if (p + 1 < L && !carry_next) load_row<T, NP>(w_out, sen[p + 1], ld, nq, lane, vnext);
+            if (np < L && np_slot < 0) load_row<T, NP>(w_in, sen[np], ld, nq, lane, pf);
             for (int i = b; i < 2 * W + 1 - b; i++) {
                 if (i == W) continue;
                 const int c = p - W + i;
                 if (c < 0) continue;
                 if (c >= L) break;
-                const int slot = c % R;
+                const int slot = pos_slot[c % R];
                 float2 u[NP];
                 float part = 0.0f;
 #pragma unroll
@@ -166,7 +228,7 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
                     part += u[t].x * v[t].x + u[t].y * v[t].y;
                 }
                 const float f = wave_sum(part);
-                const float g = grad_label1(f, alpha, et);
+                const float g = grad_label1(f, alpha_p, et);
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
                     const int q = lane + kWave * t;
@@ -177,25 +239,60 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
                 cnt[kCntPairs]++;
                 if (a.compute_loss) loss_a += (double)softplusf(-f);
             }
-            store_delta<T, NP>(w_out, sen[p], ld, nq, lane, v, v0, salt0 ^ hash32((uint32_t)p));
-            if (np < L) {
-                if (np - R >= 0) flush(np - R);
-                install(np, pf);
+            if (!carry_next)
+                store_delta<T, NP>(w_out, sen[p], ld, nq, lane, v, v0, salt0 ^ hash32((uint32_t)p), a.rounding);
+            carried = carry_next;
+            // position p-W leaves the window; position np enters it
+            const int r = p - W;
+            const int r_slot = r >= 0 ? pos_slot[r % R] : -1;
+            if (np < L && np_slot >= 0) bind(np, np_slot, sen[np]);
+            if (r_slot >= 0 && slot_last[r_slot] == r) flush(r_slot);
+            if (np < L && np_slot < 0) {
+                const int sl = find_free(p + 1 - W);
+                fill(sl, pf);
+                bind(np, sl, sen[np]);
             }
         }
-        const int lo = L - R > 0 ? L - R : 0;
-        for (int c = lo; c < L; c++) flush(c);
+        for (int r0 = 0; r0 < R; r0 += kWave) {  // slots still referenced by the last window
+            const int sx = r0 + lane;
+            uint64_t mask = __builtin_amdgcn_ballot_w64(sx < R && slot_last[sx] >= L - W);
+            while (mask) {
+                const int sl = r0 + (int)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                flush(sl);
+            }
+        }
     }
 
-    // ---- repulsion
+    // ---- repulsion (cpu:571-679)
+    // Per batch of B positions: K shared negatives; logits G = N C^T against
+    // the batch rows C, label-0 gradient, then W_out[n_k] += sum_i G_ki c_i and
+    // W_in[c_i] += sum_k G_ki n_k, all from one snapshot as the reference's
+    // three cblas_sgemm calls (cpu:633-656).  The batch rows are snapshotted
+    // into the (now idle) ring memory; repeated negatives are merged with a
+    // multiplicity so every word's W_out row is read and written by exactly
+    // one register block of NB negatives.
     const int B = a.B, NEG = a.NEG;
     const int nbat = (L + B - 1) / B;
+    const float alpha_r = L == 1 ? alpha0 : alpha;
+    PT *snap = reinterpret_cast<PT *>(lds);
+    int32_t *uneg = pos_slot + R;   // unique negatives of the batch
+    int32_t *umul = uneg + a.kmax;  // their multiplicities
     for (int bi = 0; bi < nbat; bi++) {
         const int lo = bi * B;
         const int isz = L - lo < B ? L - lo : B;
-        const uint32_t r = draw(a.seed, s, ep, kTagRepWindow, (uint32_t)bi).x;
-        const int rep_b = (lo <= W - 1 || (L - 1) - lo <= W - 1) ? (int)(r % (uint32_t)W) : (int)(r % (uint32_t)(2 * W));
-        const int K = rep_b * NEG;
+        int K;
+        int64_t n0 = 0;
+        if (planned) {
+            const int64_t bo = a.plan_batch_off[s] + bi;
+            n0 = a.plan_neg_off[bo];
+            K = (int)(a.plan_neg_off[bo + 1] - n0);
+        } else {
+            const uint32_t r = draw(a.seed, s, ep, kTagRepWindow, (uint32_t)bi).x;
+            const int rep_b =
+                (lo <= W - 1 || (L - 1) - lo <= W - 1) ? (int)(r % (uint32_t)W) : (int)(r % (uint32_t)(2 * W));
+            K = rep_b * NEG;
+        }
         cnt[kCntBatches]++;
         cnt[kCntNegatives] += (uint64_t)K;
         cnt[kCntRepPairs] += (uint64_t)K * (uint64_t)isz;
@@ -204,6 +301,10 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
         for (int k0 = 0; k0 < K; k0 += kWave) {
             const int k = k0 + lane;
             const bool live = k < K;
+            if (planned) {
+                if (live) negs[k] = a.plan_negs[n0 + k];
+                continue;
+            }
             const uint32_t idx = (uint32_t)bi * 8192u + (uint32_t)k;
             U4 rr = draw(a.seed, s, ep, kTagNeg, idx);
             int cand;
@@ -232,18 +333,46 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
             }
             if (live) negs[k] = cand;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
+        // merge repeated negatives: first occurrence keeps the word, counts copies
+        int U = 0;
+        for (int k0 = 0; k0 < K; k0 += kWave) {
+            const int k = k0 + lane;
+            const int w = k < K ? negs[k] : -1;
+            bool first = k < K;
+            int m = 0;
+            for (int j = 0; j < K; j++) {
+                const int wj = negs[j];
+                m += wj == w;
+                first = first && !(wj == w && j < k);
+            }
+            const uint64_t fm = __builtin_amdgcn_ballot_w64(first);
+            const int before = __builtin_amdgcn_mbcnt_hi((uint32_t)(fm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)fm, 0));
+            if (first) {
+                uneg[U + before] = w;
+                umul[U + before] = m;
+            }
+            U += __builtin_popcountll(fm);
+        }
+        // snapshot of the batch rows (after this sentence's attraction)
+        for (int i = 0; i < isz; i++) {
+            PT row[NP];
+            load_row<T, NP>(w_in, sen[lo + i], ld, nq, lane, row);
+#pragma unroll
+            for (int t = 0; t < NP; t++) {
+                const int q = lane + kWave * t;
+                if (q < nq) snap[i * nq + q] = row[t];
+            }
+        }
         const uint32_t salt_b = salt0 ^ hash32((uint32_t)bi * 0x27D4EB2Fu + 0x5000u);
-        for (int kb = 0; kb < K; kb += NB) {
-            const int nbk = K - kb < NB ? K - kb : NB;
+        for (int kb = 0; kb < U; kb += NB) {
+            const int nbk = U - kb < NB ? U - kb : NB;
             int32_t nrow[NB];
+            float mult[NB];
             float2 nv[NB][NP], dn[NB][NP];
 #pragma unroll
             for (int k = 0; k < NB; k++) {
-                nrow[k] = k < nbk ? negs[kb + k] : -1;
+                nrow[k] = k < nbk ? uneg[kb + k] : -1;
+                mult[k] = k < nbk ? (float)umul[kb + k] : 0.0f;
                 PT tmp[NP];
                 if (k < nbk) load_row<T, NP>(w_out, nrow[k], ld, nq, lane, tmp);
                 else {
@@ -254,13 +383,13 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
 #pragma unroll
                 for (int t = 0; t < NP; t++) dn[k][t] = make_float2(0.0f, 0.0f);
             }
-            PT cn[NP];
-            load_row<T, NP>(w_in, sen[lo], ld, nq, lane, cn);
             for (int i = 0; i < isz; i++) {
                 float2 c[NP];
-                to_f<T, NP>(cn, c);
-                const int64_t crow = sen[lo + i];
-                if (i + 1 < isz) load_row<T, NP>(w_in, sen[lo + i + 1], ld, nq, lane, cn);
+#pragma unroll
+                for (int t = 0; t < NP; t++) {
+                    const int q = lane + kWave * t;
+                    c[t] = q < nq ? Pair<T>::tof(snap[i * nq + q]) : make_float2(0.0f, 0.0f);
+                }
                 float g[NB];
 #pragma unroll
                 for (int k = 0; k < NB; k++) {
@@ -268,8 +397,8 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
 #pragma unroll
                     for (int t = 0; t < NP; t++) part += nv[k][t].x * c[t].x + nv[k][t].y * c[t].y;
                     const float f = wave_sum(part);
-                    g[k] = k < nbk ? grad_label0(f, alpha, et) : 0.0f;
-                    if (a.compute_loss && k < nbk) loss_r += (double)softplusf(f);
+                    g[k] = k < nbk ? grad_label0(f, alpha_r, et) : 0.0f;
+                    if (a.compute_loss && k < nbk) loss_r += (double)mult[k] * (double)softplusf(f);
                 }
                 float2 dc[NP];
 #pragma unroll
@@ -277,8 +406,9 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
                     dc[t] = make_float2(0.0f, 0.0f);
 #pragma unroll
                     for (int k = 0; k < NB; k++) {
-                        dc[t].x += g[k] * nv[k][t].x;
-                        dc[t].y += g[k] * nv[k][t].y;
+                        const float gm = g[k] * mult[k];
+                        dc[t].x += gm * nv[k][t].x;
+                        dc[t].y += gm * nv[k][t].y;
                         dn[k][t].x += g[k] * c[t].x;
                         dn[k][t].y += g[k] * c[t].y;
                     }
@@ -286,15 +416,21 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
                 float2 zero[NP];
 #pragma unroll
                 for (int t = 0; t < NP; t++) zero[t] = make_float2(0.0f, 0.0f);
-                store_delta<T, NP>(w_in, crow, ld, nq, lane, dc, zero, salt_b ^ hash32((uint32_t)(kb * 1024 + i)));
+                store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, dc, zero,
+                                   salt_b ^ hash32((uint32_t)(kb * 1024 + i)), a.rounding);
             }
 #pragma unroll
             for (int k = 0; k < NB; k++) {
                 if (k < nbk) {
                     float2 zero[NP];
 #pragma unroll
-                    for (int t = 0; t < NP; t++) zero[t] = make_float2(0.0f, 0.0f);
-                    store_delta<T, NP>(w_out, nrow[k], ld, nq, lane, dn[k], zero, salt_b ^ hash32((uint32_t)(kb + k) + 0x9000u));
+                    for (int t = 0; t < NP; t++) {
+                        zero[t] = make_float2(0.0f, 0.0f);
+                        dn[k][t].x *= mult[k];
+                        dn[k][t].y *= mult[k];
+                    }
+                    store_delta<T, NP>(w_out, nrow[k], ld, nq, lane, dn[k], zero,
+                                       salt_b ^ hash32((uint32_t)(kb + k) + 0x9000u), a.rounding);
                 }
             }
         }

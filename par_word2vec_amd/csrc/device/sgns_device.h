@@ -100,17 +100,23 @@ __device__ __forceinline__ uint32_t bf16_rne(float x) {
 // ------------------------------------------------------------- row pairs
 // Storage traits: a "pair" is two adjacent elements of a row.
 template <typename T> struct Pair;
+// Table rows are read with agent-coherent loads (sc1 on gfx950): every
+// update reaches memory through the L2 (float atomics or coherent stores), so
+// an ordinary load could keep returning a row's stale copy from the CU's
+// vector L1 for as long as that row stays hot there.
 template <> struct Pair<float> {
     using P = float2;
     __device__ static __forceinline__ float2 load(const float *w, int64_t row, int ld, int q) {
-        return reinterpret_cast<const float2 *>(w + row * ld)[q];
+        const uint64_t *p = reinterpret_cast<const uint64_t *>(w + row * ld) + q;
+        return __builtin_bit_cast(float2, __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
     __device__ static __forceinline__ float2 tof(float2 p) { return p; }
 };
 template <> struct Pair<uint16_t> {  // bf16 bit patterns
     using P = uint32_t;
     __device__ static __forceinline__ uint32_t load(const uint16_t *w, int64_t row, int ld, int q) {
-        return reinterpret_cast<const uint32_t *>(w + row * ld)[q];
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(w + row * ld) + q;
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ static __forceinline__ float2 tof(uint32_t p) { return make_float2(bf16lo(p), bf16hi(p)); }
 };

@@ -39,6 +39,7 @@ struct FastArgs {
     int32_t ld, d, nq, W, R, NEG, B, V, kmax;
     int32_t lds_per_wave;         // bytes
     int32_t persistent;
+    int32_t rounding;             // bf16 write-back: 0 stochastic, 1 nearest
     int32_t compute_loss;
     int32_t iter, epoch;
     float a0;
@@ -47,6 +48,13 @@ struct FastArgs {
     unsigned long long *queue;
     double *loss;
     unsigned long long *counters;
+    // optional exact schedule (window shrink per centre, learning rate of
+    // centre 0 / the rest, negatives per batch) replacing the Philox draws
+    const uint8_t *plan_b;
+    const float *plan_alpha;
+    const int64_t *plan_batch_off;
+    const int64_t *plan_neg_off;
+    const int32_t *plan_negs;
 };
 
 struct WindowArgs {
@@ -65,9 +73,16 @@ hipError_t launch_reset(unsigned long long *counters, int n_counters, double *lo
 hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int flags, hipStream_t st);
 
 // per-wave LDS bytes of the fast kernel
-inline int fast_lds_per_wave(int R, int nq, int bf16, int kmax) {
-    int ring = R * nq * (8 + (bf16 ? 4 : 8));
-    return ((ring + kmax * 4) + 15) & ~15;
+// ring values + snapshots during attraction, reused as the batch-row
+// snapshot during repulsion; then negatives (drawn, unique, multiplicity)
+// and the three ring slot tables
+inline int fast_lds_per_wave(int R, int nq, int bf16, int kmax, int B) {
+    const int e = bf16 ? 4 : 8;
+    int ring = R * nq * (8 + e);
+    int snap = B * nq * e;
+    int region = ring > snap ? ring : snap;
+    region = (region + 15) & ~15;
+    return (region + 3 * kmax * 4 + 3 * R * 4 + 15) & ~15;
 }
 
 }  // namespace sgns

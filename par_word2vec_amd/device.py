@@ -14,6 +14,7 @@ DETERMINISTIC, PLAIN, PERSISTENT = 0, 1, 2
 MODES = {"det": DETERMINISTIC, "deterministic": DETERMINISTIC, "plain": PLAIN, "persistent": PERSISTENT}
 PRECISIONS = {"fp32": FP32, "bf16": BF16}
 WIN_MFMA, WIN_FP64 = 1, 2
+ROUNDINGS = {"stochastic": 0, "nearest": 1}
 
 
 class DeviceError(RuntimeError):
@@ -22,10 +23,10 @@ class DeviceError(RuntimeError):
 
 class SgnsDevice:
     def __init__(self, vocab_size: int, dim: int, window: int, negative: int, batch_size: int,
-                 precision: str = "fp32", device: int = 0):
+                 precision: str = "fp32", device: int = 0, rounding: str = "stochastic"):
         self.lib = N.sgns()
         self.cfg = N.ParSgnsConfig(vocab_size, dim, window, negative, batch_size, PRECISIONS[precision],
-                                   device, 0)
+                                   device, ROUNDINGS[rounding])
         self.vocab_size, self.dim = vocab_size, dim
         self.precision = precision
         h = C.c_void_p()
@@ -101,10 +102,10 @@ class SgnsDevice:
     # ------------------------------------------------------------- training
     def train(self, mode: str | int, sent_begin: int, sent_end: int, alpha: float, train_words: int,
               iters: int = 1, epoch: int = 0, seed: int = 1, compute_loss: bool = False,
-              waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False) -> dict:
+              waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False, planned: bool = False) -> dict:
         m = MODES[mode] if isinstance(mode, str) else int(mode)
         args = N.ParSgnsTrainArgs(m, iters, epoch, int(compute_loss), alpha, waves_per_block, blocks,
-                                  int(use_graph), train_words, seed, sent_begin, sent_end)
+                                  int(use_graph), int(planned), 0, train_words, seed, sent_begin, sent_end)
         st = N.ParSgnsStats()
         self._chk(self.lib.par_sgns_train(self._h, C.byref(args), C.byref(st)), "train")
         return st.as_dict()

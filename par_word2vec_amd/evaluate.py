@@ -17,11 +17,13 @@ def _topk_neighbours(w: np.ndarray, queries: np.ndarray, k: int, candidates: np.
     """Indices (into `candidates`, or into rows of w) of the k most cosine-
     similar rows for each query row index, excluding the query itself."""
     cand = np.arange(w.shape[0]) if candidates is None else candidates
-    try:
-        import torch
-        use_gpu = torch.cuda.is_available()
-    except Exception:  # noqa: BLE001 - torch absent means the numpy path
-        use_gpu = False
+    use_gpu = False
+    if cand.size * queries.size > 2e8:  # large problems only: small ones stay in numpy
+        try:
+            import torch
+            use_gpu = torch.cuda.is_available()
+        except Exception:  # noqa: BLE001 - torch absent means the numpy path
+            use_gpu = False
     norms = np.linalg.norm(w, axis=1, keepdims=True)
     wn = (w / np.maximum(norms, 1e-12)).astype(np.float32)
     out = np.empty((queries.size, k), np.int64)
