@@ -105,6 +105,11 @@ int par_sgns_set_exp_table(par_sgns_handle h, const float *table, int32_t n);
  * does.  Replaces the 400 MB d_table upload (text.cu:112-113). */
 int par_sgns_set_sampler(par_sgns_handle h, const int64_t *entries, int32_t vocab_size);
 
+/* n draws of the on-device negative sampler (alias table + Philox, before
+ * in-sentence rejection), keyed like the trainer's draws for sentence ids
+ * sent, sent+1, ... (8192 draws each).  For checking the distribution. */
+int par_sgns_sample_negatives(par_sgns_handle h, uint64_t seed, int64_t sent, int64_t n, int32_t *out);
+
 /* Weights (text.cu:371-374 upload, text.cu:963 download).  Host arrays are
  * [V][dim] float32; NULL skips that table.  BF16 handles round to nearest on
  * upload. */

@@ -93,6 +93,7 @@ SGNS_SYMBOLS = {
     "par_sgns_row_stride": (i32, [vp]),
     "par_sgns_set_exp_table": (C.c_int, [vp, vp, i32]),
     "par_sgns_set_sampler": (C.c_int, [vp, vp, i32]),
+    "par_sgns_sample_negatives": (C.c_int, [vp, u64, i64, i64, vp]),
     "par_sgns_set_weights": (C.c_int, [vp, vp, vp]),
     "par_sgns_get_weights": (C.c_int, [vp, vp, vp]),
     "par_sgns_set_corpus": (C.c_int, [vp, vp, i64, vp, vp, i64]),

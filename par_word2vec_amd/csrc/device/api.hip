@@ -52,6 +52,8 @@ struct par_sgns_handle_s {
     int32_t *tokens = nullptr;
     int64_t *sent_off = nullptr, *sent_raw = nullptr;
     int32_t *order = nullptr;
+    int64_t *queue_raw = nullptr;
+    std::vector<int64_t> h_sent_raw;
     int64_t n_tokens = 0, n_sent = 0;
     std::vector<int64_t> h_sent_off;
     std::set<std::pair<int64_t, int64_t>> ordered;
@@ -202,7 +204,7 @@ int par_sgns_destroy(par_sgns_handle h) {
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
     dfree(h->w_in); dfree(h->w_out); dfree(h->exp_table); dfree(h->alias_prob); dfree(h->alias_idx);
-    dfree(h->tokens); dfree(h->sent_off); dfree(h->sent_raw); dfree(h->order);
+    dfree(h->tokens); dfree(h->sent_off); dfree(h->sent_raw); dfree(h->order); dfree(h->queue_raw);
     dfree(h->p_tokens); dfree(h->p_negs); dfree(h->p_b); dfree(h->p_sent_off); dfree(h->p_batch_off);
     dfree(h->p_neg_off); dfree(h->p_alpha); dfree(h->det_scratch);
     dfree(h->counters); dfree(h->queue); dfree(h->loss); dfree(h->g_in); dfree(h->g_out); dfree(h->win_buf);
@@ -277,6 +279,22 @@ int par_sgns_set_sampler(par_sgns_handle h, const int64_t *entries, int32_t V) {
     return 0;
 }
 
+int par_sgns_sample_negatives(par_sgns_handle h, uint64_t seed, int64_t sent, int64_t n, int32_t *out) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!out || n < 0 || sent < 0) return seterr(h, PAR_SGNS_EINVAL, "bad sample arguments");
+    if (!h->have_sampler) return seterr(h, PAR_SGNS_ESTATE, "sampler not set");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int32_t *d = nullptr;
+    int rc = dalloc(h, d, (size_t)n);
+    if (rc) return rc;
+    hipError_t e = sgns::launch_sample(h->alias_prob, h->alias_idx, h->cfg.vocab_size, seed, sent, n, d, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)n * 4, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    dfree(d);
+    if (e != hipSuccess) return seterr(h, PAR_SGNS_EDEVICE, "sampling failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int par_sgns_set_weights(par_sgns_handle h, const float *w_in, const float *w_out) {
     if (!h) return PAR_SGNS_EINVAL;
     const int64_t V = h->cfg.vocab_size, d = h->cfg.dim, ld = h->ld;
@@ -346,8 +364,10 @@ int par_sgns_set_corpus(par_sgns_handle h, const int32_t *tokens, int64_t n_toke
     std::vector<int32_t> ident((size_t)std::max<int64_t>(n_sent, 1));
     std::iota(ident.begin(), ident.end(), 0);
     if ((rc = upload(h, h->order, ident.data(), ident.size()))) return rc;
+    if ((rc = upload(h, h->queue_raw, sent_raw, (size_t)std::max<int64_t>(n_sent, 1)))) return rc;
     HIPCHK(h, hipStreamSynchronize(h->st));
     h->h_sent_off.assign(sent_off, sent_off + n_sent + 1);
+    h->h_sent_raw.assign(sent_raw, sent_raw + n_sent);
     h->n_tokens = n_tokens;
     h->n_sent = n_sent;
     h->ordered.clear();
@@ -405,13 +425,30 @@ static int ensure_order(par_sgns_handle h, int64_t b, int64_t e) {
     std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
         return off[x + 1] - off[x] > off[y + 1] - off[y];
     });
+    // training progress at each queue slot: raw words of the sentences dequeued
+    // before it (a sentence's raw span runs to the next sentence's start)
+    const int64_t *raw = h->h_sent_raw.data();
+    std::vector<int64_t> prog(ord.size());
+    int64_t acc = raw[b];
+    for (size_t k = 0; k < ord.size(); k++) {
+        prog[k] = acc;
+        const int32_t x = ord[k];
+        acc += x + 1 < h->n_sent ? raw[x + 1] - raw[x] : (off[x + 1] - off[x]);
+    }
     HIPCHK(h, hipMemcpyAsync(h->order + b, ord.data(), ord.size() * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipMemcpyAsync(h->queue_raw + b, prog.data(), prog.size() * sizeof(int64_t), hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
     h->ordered.insert({b, e});
     return 0;
 }
 
-static void default_shape(par_sgns_handle h, int lds_per_wave, int &wpb, int &blocks) {
+// Default persistent grid: as many resident waves as the LDS allows, but at
+// least kMinSentencesPerWave sentences per wave.  Hogwild quality depends on
+// each row seeing a sequence of updates; a small corpus spread over every
+// resident wave of the chip turns a pass into a few huge stale steps.
+constexpr int64_t kMinSentencesPerWave = 16;
+
+static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, int &wpb, int &blocks) {
     if (wpb <= 0) {
         wpb = 4;
         while (wpb > 1 && (size_t)lds_per_wave * wpb > 64 * 1024) wpb /= 2;
@@ -423,6 +460,8 @@ static void default_shape(par_sgns_handle h, int lds_per_wave, int &wpb, int &bl
         int bpc = waves_cu / wpb;
         if (bpc < 1) bpc = 1;
         blocks = bpc * (h->cu_count > 0 ? h->cu_count : 256);
+        const int64_t cap = n_sent / kMinSentencesPerWave / wpb;
+        if (blocks > cap) blocks = (int)(cap < 1 ? 1 : cap);
     }
 }
 
@@ -470,6 +509,7 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
         const bool pers = a->mode == PAR_SGNS_PERSISTENT;
         sgns::FastArgs f{};
         f.tokens = h->tokens; f.sent_off = h->sent_off; f.sent_raw = h->sent_raw; f.order = pers ? h->order : nullptr;
+        f.queue_raw = h->queue_raw;
         if (planned) {
             // exact schedule: plan tokens in plan order (sentence order is kept so a
             // single wave replays the reference's sequence)
@@ -486,7 +526,7 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
         f.a0 = a->alpha; f.T = a->train_words; f.seed = a->seed; f.s_begin = a->sent_begin; f.s_end = a->sent_end;
         f.queue = h->queue; f.loss = h->loss; f.counters = h->counters;
         int wpb = pers ? a->waves_per_block : 1, blocks = a->blocks;
-        default_shape(h, f.lds_per_wave, wpb, blocks);
+        default_shape(h, f.lds_per_wave, a->sent_end - a->sent_begin, wpb, blocks);
         if (!pers) wpb = a->waves_per_block > 0 ? a->waves_per_block : 1;
         if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);

@@ -85,10 +85,21 @@ __device__ __forceinline__ void store_delta(T *w, int64_t row, int ld, int nq, i
     }
 }
 
+// One draw from the unigram^0.75 alias table (before in-sentence rejection).
+__device__ __forceinline__ int alias_draw(const float *prob, const int32_t *alias, int V, U4 rr) {
+    const uint32_t bucket = __umulhi(rr.x, (uint32_t)V);
+    const float u2 = (float)(rr.y >> 8) * (1.0f / 16777216.0f);
+    return u2 < prob[bucket] ? (int)bucket : alias[bucket];
+}
+
 // ------------------------------------------------------ one sentence
+// `raw` is the training progress in raw words when the sentence starts: its
+// corpus offset in sentence order, or the words dequeued before it when a
+// queue reorders sentences -- the reference decays alpha with the words
+// processed so far (word_count_actual, cpu:431-445), not with corpus position.
 template <typename T, int NP, int NB>
-__device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds, double &loss_a, double &loss_r,
-                               uint64_t (&cnt)[kCntSlots], const float *et) {
+__device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int lane, char *lds, double &loss_a,
+                               double &loss_r, uint64_t (&cnt)[kCntSlots], const float *et) {
     using PT = typename Pair<T>::P;
     T *w_in = static_cast<T *>(a.w_in);
     T *w_out = static_cast<T *>(a.w_out);
@@ -98,7 +109,7 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
     if (L <= 0) return;  // an empty sentence trains nothing (cpu:539-542, 575)
     const int32_t *sen = a.tokens + o0;
     const bool planned = a.plan_b != nullptr;
-    const float alpha = planned ? a.plan_alpha[2 * s + 1] : sentence_alpha(a.a0, a.epoch, a.T, a.iter, a.sent_raw[s]);
+    const float alpha = planned ? a.plan_alpha[2 * s + 1] : sentence_alpha(a.a0, a.epoch, a.T, a.iter, raw);
     const float alpha0 = planned ? a.plan_alpha[2 * s] : alpha;  // centre 0 (cpu:431-447 timing)
     const uint32_t ep = (uint32_t)a.epoch;
     const uint32_t salt0 = hash32((uint32_t)a.seed ^ hash32((uint32_t)s * 0x632BE5ABu ^ (ep << 24)));
@@ -306,13 +317,7 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int lane, char *lds
                 continue;
             }
             const uint32_t idx = (uint32_t)bi * 8192u + (uint32_t)k;
-            U4 rr = draw(a.seed, s, ep, kTagNeg, idx);
-            int cand;
-            {
-                const uint32_t bucket = __umulhi(rr.x, (uint32_t)a.V);
-                const float u2 = (float)(rr.y >> 8) * (1.0f / 16777216.0f);
-                cand = u2 < a.alias_prob[bucket] ? (int)bucket : a.alias_idx[bucket];
-            }
+            int cand = alias_draw(a.alias_prob, a.alias_idx, a.V, draw(a.seed, s, ep, kTagNeg, idx));
             // Redraw uniformly over [1, V-1] while the candidate occurs in the
             // sentence.  The reference loops without bound (cpu:611-615); the
             // bound here only matters when the sentence holds nearly the whole
@@ -452,12 +457,15 @@ __global__ __launch_bounds__(256) void sgns_fast_kernel(FastArgs a) {
             if (lane == 0) qi = atomicAdd(a.queue, 1ull);
             const int64_t idx = readfirst_i64((int64_t)qi);
             if (idx >= n) break;
-            const int64_t s = a.order ? (int64_t)a.order[a.s_begin + idx] : a.s_begin + idx;
-            train_sentence<T, NP, NB>(a, s, lane, lds, loss_a, loss_r, cnt, et);
+            const int64_t slot = a.s_begin + idx;
+            const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
+            const int64_t raw = a.plan_b ? 0 : (a.order ? a.queue_raw[slot] : a.sent_raw[s]);
+            train_sentence<T, NP, NB>(a, s, raw, lane, lds, loss_a, loss_r, cnt, et);
         }
     } else {
         const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
-        if (idx < n) train_sentence<T, NP, NB>(a, a.s_begin + idx, lane, lds, loss_a, loss_r, cnt, et);
+        const int64_t s = a.s_begin + idx;
+        if (idx < n) train_sentence<T, NP, NB>(a, s, a.plan_b ? 0 : a.sent_raw[s], lane, lds, loss_a, loss_r, cnt, et);
     }
     if (lane == 0) {
         if (a.compute_loss) {
@@ -468,6 +476,14 @@ __global__ __launch_bounds__(256) void sgns_fast_kernel(FastArgs a) {
         for (int k = 0; k < kCntRepPairs + 1; k++)
             if (cnt[k]) atomicAdd(&a.counters[k], (unsigned long long)cnt[k]);
     }
+}
+
+// n draws of the sampler for sentence id `sent` (the counter layout of the
+// trainer's negatives), for testing the on-device distribution.
+__global__ void sgns_sample_kernel(const float *prob, const int32_t *alias, int V, uint64_t seed, int64_t sent,
+                                   int64_t n, int32_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = alias_draw(prob, alias, V, draw(seed, sent + (i >> 13), 0, kTagNeg, (uint32_t)(i & 8191)));
 }
 
 __global__ void sgns_reset_kernel(unsigned long long *counters, int n, double *loss, unsigned long long *queue) {
@@ -512,6 +528,14 @@ hipError_t launch_fast(const FastArgs &a, int bf16, int np, int wpb, int blocks,
         }
     }
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_sample(const float *prob, const int32_t *alias, int V, uint64_t seed, int64_t sent, int64_t n,
+                         int32_t *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sgns_sample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, prob, alias, V, seed,
+                       sent, n, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset(unsigned long long *counters, int n, double *loss, unsigned long long *queue, hipStream_t st) {

@@ -32,6 +32,7 @@ struct FastArgs {
     const int64_t *sent_off;
     const int64_t *sent_raw;
     const int32_t *order;         // persistent queue order (may be null)
+    const int64_t *queue_raw;     // raw words dequeued before each queue slot (with order)
     const float *exp_table;
     const float *alias_prob;
     const int32_t *alias_idx;
@@ -70,6 +71,8 @@ hipError_t launch_det(const DetArgs &a, hipStream_t st);
 hipError_t launch_fast(const FastArgs &a, int bf16, int np, int waves_per_block, int blocks, hipStream_t st);
 hipError_t launch_reset(unsigned long long *counters, int n_counters, double *loss, unsigned long long *queue,
                         hipStream_t st);
+hipError_t launch_sample(const float *prob, const int32_t *alias, int V, uint64_t seed, int64_t sent, int64_t n,
+                         int32_t *out, hipStream_t st);
 hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int flags, hipStream_t st);
 
 // per-wave LDS bytes of the fast kernel

@@ -69,6 +69,11 @@ class SgnsDevice:
         e = np.ascontiguousarray(entries, dtype=np.int64)
         self._chk(self.lib.par_sgns_set_sampler(self._h, N.ptr(e), e.size), "set_sampler")
 
+    def sample_negatives(self, n: int, seed: int = 1, sent: int = 0) -> np.ndarray:
+        out = np.empty(n, dtype=np.int32)
+        self._chk(self.lib.par_sgns_sample_negatives(self._h, seed, sent, n, N.ptr(out)), "sample_negatives")
+        return out
+
     def set_weights(self, w_in: np.ndarray | None, w_out: np.ndarray | None) -> None:
         a = None if w_in is None else np.ascontiguousarray(w_in, dtype=np.float32)
         b = None if w_out is None else np.ascontiguousarray(w_out, dtype=np.float32)

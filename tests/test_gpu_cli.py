@@ -1,0 +1,52 @@
+"""End to end through the drop-in executable: text corpus in, embedding file
+out, compared with the oracle's training + saveModel (cpu:721-737)."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+from par_word2vec_amd import host as H
+
+pytestmark = pytest.mark.gpu
+CLI = Path(__file__).resolve().parent.parent / "par_word2vec_amd" / "lib" / "par_word2vec"
+
+
+def _corpus(tmp_path):
+    raw, _ = H.synth_zipf(80_000, 2000, 1.0, line_len=300, n_clusters=5, cluster_size=10, p_replace=0.3, seed=21)
+    p = str(tmp_path / "corpus.txt")
+    H.write_raw_text(p, raw)
+    return p
+
+
+def test_cli_det_output_bytes_equal_oracle(tmp_path):
+    p = _corpus(tmp_path)
+    out = tmp_path / "vec.bin"
+    flags = ["-train", p, "-output", str(out), "-size", "64", "-window", "5", "-negative", "5", "-sample", "1e-3",
+             "-iter", "2", "-min-count", "5", "-batch-size", "24", "-binary", "1", "-mode", "det",
+             "-time-file", str(tmp_path / "t")]
+    r = subprocess.run([str(CLI), *flags], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    ov = O.OracleVocab.learn(p, 5)
+    st = ov.stream(p)
+    res = O.train(st, ov.counts(), ov.train_words, 64, 5, 5, 24, 2, 0.025, 1e-3)
+    ov.save_model(str(tmp_path / "ref.bin"), res.w_in, True)
+    assert out.read_bytes() == (tmp_path / "ref.bin").read_bytes()
+    assert (tmp_path / "t").read_text().startswith("Elapsed ")
+
+
+def test_cli_persistent_text_output(tmp_path):
+    p = _corpus(tmp_path)
+    out = tmp_path / "vec.txt"
+    r = subprocess.run([str(CLI), "-train", p, "-output", str(out), "-size", "32", "-iter", "2", "-precision",
+                        "bf16", "-sample", "1e-4", "-save-vocab", str(tmp_path / "v.txt"),
+                        "-time-file", str(tmp_path / "t")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().split("\n")
+    V, d = map(int, lines[0].split())
+    assert d == 32 and V == len(H.Vocab.learn_text(p, 5))
+    assert lines[1].startswith("</s> ")
+    vals = np.array(lines[2].split()[1:], dtype=np.float64)
+    assert vals.size == 32 and np.isfinite(vals).all()
+    assert (tmp_path / "v.txt").read_text().startswith("</s> ")

@@ -1,0 +1,133 @@
+"""Loss-forward and adjoint kernels on the 65,536-window workload: windows of
+one centre, 2x5 contexts and 5 shared negatives over a 100,000 x 300 table,
+both tables U(-0.5/300, 0.5/300).
+
+Bars:
+  forward, FP32 accumulation   within 1e-6 relative of the FP64 oracle
+  forward, FP64 accumulation   within 1e-11 relative of the FP64 oracle
+  adjoint (FP32 atomics)       within 1e-5 of max|grad| of the FP64 oracle gradient
+  adjoint vs finite differences   FP64 central differences (h = 1e-4) on 1,000
+                               random coordinates of referenced rows, 1e-4
+                               relative + 1e-7 absolute
+  adjoint vs the CPU pair update  -alpha * dL/dW equals the oracle's
+                               attraction update (cpu:551-565) up to the
+                               sigmoid table's quantisation (1% relative)
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from par_word2vec_amd.device import SgnsDevice, WIN_FP64
+
+pytestmark = pytest.mark.gpu
+
+V, D, NW, NCTX, NNEG = 100_000, 300, 65_536, 10, 5
+
+
+def bf16_round(x):
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32)
+    b = (b + np.uint32(0x7FFF) + ((b >> np.uint32(16)) & np.uint32(1))) & np.uint32(0xFFFF0000)
+    return b.view(np.float32)
+
+
+@pytest.fixture(scope="module")
+def windows():
+    rng = np.random.default_rng(2024)
+    w_in = rng.uniform(-0.5 / D, 0.5 / D, (V, D)).astype(np.float32)
+    w_out = rng.uniform(-0.5 / D, 0.5 / D, (V, D)).astype(np.float32)
+    centre = rng.integers(1, V, NW).astype(np.int32)
+    ctx = rng.integers(1, V, (NW, NCTX)).astype(np.int32)
+    neg = rng.integers(1, V, (NW, NNEG)).astype(np.int32)
+    return w_in, w_out, centre, ctx, neg
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_forward_matches_fp64_oracle(windows, precision):
+    w_in, w_out, centre, ctx, neg = windows
+    if precision == "bf16":
+        w_in, w_out = bf16_round(w_in), bf16_round(w_out)
+    ref = O.window_loss_f64(centre, ctx, neg, w_in, w_out)
+    with SgnsDevice(V, D, 5, NNEG, 24, precision) as dev:
+        dev.set_weights(w_in, w_out)
+        l32 = dev.loss_forward(centre, ctx, neg)
+        l64 = dev.loss_forward(centre, ctx, neg, flags=WIN_FP64)
+    print(f"[{precision}] loss fp32 {l32:.9f} fp64 {l64:.12f} oracle {ref:.12f}", flush=True)
+    assert l32 == pytest.approx(ref, rel=1e-6)
+    assert l64 == pytest.approx(ref, rel=1e-11)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_adjoint_matches_fp64_gradient(windows, precision):
+    w_in, w_out, centre, ctx, neg = windows
+    if precision == "bf16":
+        w_in, w_out = bf16_round(w_in), bf16_round(w_out)
+    gi_ref, go_ref = O.window_grad_f64(centre, ctx, neg, w_in, w_out)
+    with SgnsDevice(V, D, 5, NNEG, 24, precision) as dev:
+        dev.set_weights(w_in, w_out)
+        loss = dev.loss_adjoint(centre, ctx, neg)
+        gi, go = dev.get_grads()
+        ms, _ = dev.last_kernel_ms(2)
+    print(f"[{precision}] adjoint {ms:.3f} ms; max|dgi| {np.abs(gi - gi_ref).max():.3e} of {np.abs(gi_ref).max():.3e}",
+          flush=True)
+    assert loss == pytest.approx(O.window_loss_f64(centre, ctx, neg, w_in, w_out), rel=1e-6)
+    assert np.abs(gi - gi_ref).max() <= 1e-5 * np.abs(gi_ref).max()
+    assert np.abs(go - go_ref).max() <= 1e-5 * np.abs(go_ref).max()
+
+
+def test_adjoint_finite_differences(windows):
+    """FP64 central differences of the oracle's loss restricted to the
+    windows that touch the perturbed row (all other terms cancel)."""
+    w_in, w_out, centre, ctx, neg = windows
+    with SgnsDevice(V, D, 5, NNEG, 24, "fp32") as dev:
+        dev.set_weights(w_in, w_out)
+        dev.loss_adjoint(centre, ctx, neg)
+        gi, go = dev.get_grads()
+    rng = np.random.default_rng(7)
+    h = 1e-4
+    a = w_in.astype(np.float64)
+    b = w_out.astype(np.float64)
+    bad = []
+    for t in range(1000):
+        table_in = t % 2 == 0
+        if table_in:
+            r = int(ctx.reshape(-1)[rng.integers(0, ctx.size)])
+            hit = np.flatnonzero((ctx == r).any(1))
+        else:
+            pool = np.concatenate([centre, neg.reshape(-1)])
+            r = int(pool[rng.integers(0, pool.size)])
+            hit = np.flatnonzero((centre == r) | (neg == r).any(1))
+        j = int(rng.integers(0, D))
+        tab = a if table_in else b
+        old = tab[r, j]
+        tab[r, j] = old + h
+        lp = O.window_loss_f64(centre[hit], ctx[hit], neg[hit], a, b)
+        tab[r, j] = old - h
+        lm = O.window_loss_f64(centre[hit], ctx[hit], neg[hit], a, b)
+        tab[r, j] = old
+        fd = (lp - lm) / (2 * h)
+        g = float(gi[r, j] if table_in else go[r, j])
+        if abs(fd - g) > 1e-4 * abs(fd) + 1e-7:
+            bad.append((table_in, r, j, fd, g))
+    assert not bad, bad[:5]
+
+
+def test_adjoint_vs_cpu_pair_update():
+    """Sentence [a, b], window 1, no negatives: the oracle applies the pair
+    (context b, centre a) first; rows W_out[a] and W_in[b] change only there.
+    Its update equals -alpha * dL/dW of the window (centre a, context b) with
+    the table sigmoid in place of the exact one."""
+    d, alpha = 300, 0.025
+    rng = np.random.default_rng(3)
+    w_in = rng.uniform(-0.5, 0.5, (3, d)).astype(np.float32) / 8
+    w_out = rng.uniform(-0.5, 0.5, (3, d)).astype(np.float32) / 8
+    r = O.train(np.array([1, 2, 0], np.int32), np.array([1, 1, 1], np.uint32), 3, d, 1, 5, 24, 1, alpha, 0.0,
+                w_in=w_in, w_out=w_out, table=np.zeros(100_000_000, np.int32))
+    upd_out = r.w_out[1].astype(np.float64) - w_out[1]
+    upd_in = r.w_in[2].astype(np.float64) - w_in[2]
+    with SgnsDevice(3, d, 1, 0, 24, "fp32") as dev:
+        dev.set_weights(w_in, w_out)
+        dev.loss_adjoint(np.array([1], np.int32), np.array([[2]], np.int32), np.zeros((1, 0), np.int32))
+        gi, go = dev.get_grads()
+    np.testing.assert_allclose(-alpha * go[1], upd_out, rtol=1e-2, atol=1e-9)
+    np.testing.assert_allclose(-alpha * gi[2], upd_in, rtol=1e-2, atol=1e-9)
+    assert np.abs(upd_out).max() > 1e-6
