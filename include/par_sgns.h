@@ -39,6 +39,13 @@ enum par_sgns_precision { PAR_SGNS_FP32 = 0, PAR_SGNS_BF16 = 1 };
  * default; unbiased for updates below half an ulp) or round-to-nearest. */
 enum par_sgns_rounding { PAR_SGNS_ROUND_STOCHASTIC = 0, PAR_SGNS_ROUND_NEAREST = 1 };
 
+/* How plain/persistent modes evaluate a repulsion batch (the three SGEMMs
+ * of PAR_Word2Vec_cpu.cpp:633-656).  AUTO: matrix cores on bf16 tables,
+ * vector ALUs on fp32 tables.  MATRIX (bf16 tables only): G = N C^T on
+ * bf16 MFMA with fp32 sums, the gradient rounded to bf16 for the two update
+ * products.  VECTOR: fp32 dot products, register blocks of negatives. */
+enum par_sgns_repulsion { PAR_SGNS_REP_AUTO = 0, PAR_SGNS_REP_VECTOR = 1, PAR_SGNS_REP_MATRIX = 2 };
+
 enum par_sgns_mode {
     PAR_SGNS_DETERMINISTIC = 0, /* one wavefront replays an uploaded plan exactly */
     PAR_SGNS_PLAIN = 1,         /* one workgroup (one wavefront) per sentence */
@@ -50,7 +57,7 @@ typedef struct par_sgns_handle_s *par_sgns_handle;
 typedef struct par_sgns_config {
     int32_t vocab_size;   /* V, including "</s>" */
     int32_t dim;          /* -size, 1..512 */
-    int32_t window;       /* -window, 1..64 */
+    int32_t window;       /* -window, 1..31 (the ring of 2W+1 slots is one wavefront wide) */
     int32_t negative;     /* -negative, 0..64 */
     int32_t batch_size;   /* -batch-size (repulsion batch B), 1..64 */
     int32_t precision;    /* par_sgns_precision */
@@ -69,7 +76,7 @@ typedef struct par_sgns_train_args {
     int32_t use_graph;       /* persistent mode: replay a captured hipGraph */
     int32_t schedule;        /* plain/persistent: 0 = Philox draws over the uploaded
                                 corpus, 1 = the uploaded exact plan (reference draws) */
-    int32_t reserved;
+    int32_t repulsion;       /* plain/persistent: par_sgns_repulsion */
     uint64_t train_words; /* T = sum of vocabulary counts */
     uint64_t seed;        /* Philox key for windows, negatives, rounding */
     int64_t sent_begin;   /* sentence range [begin, end) of the uploaded corpus */
@@ -133,6 +140,12 @@ int par_sgns_set_plan(par_sgns_handle h, const int32_t *tokens, const uint8_t *b
 /* One training pass over a sentence range (the epoch loop body of
  * text.cu:879-952: one par_sgns_kernel launch per epoch there). */
 int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *args, par_sgns_stats *stats);
+
+/* Everything a later par_sgns_train with the same arguments needs, without
+ * running it: the longest-first queue order of the sentence range and, with
+ * use_graph, the captured hipGraph.  Lets a caller keep host work out of a
+ * timed region. */
+int par_sgns_prepare(par_sgns_handle h, const par_sgns_train_args *args);
 
 /* SGNS window-batch loss and its adjoint (no counterpart in the reference,
  * which fuses them into the update at cpu:551-565, 633-656).  Window w has

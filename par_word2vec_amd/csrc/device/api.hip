@@ -163,7 +163,7 @@ extern "C" {
 int par_sgns_create(const par_sgns_config *cfg, par_sgns_handle *out) {
     if (!cfg || !out) return PAR_SGNS_EINVAL;
     *out = nullptr;
-    if (cfg->vocab_size < 2 || cfg->dim < 1 || cfg->dim > 512 || cfg->window < 1 || cfg->window > 64 ||
+    if (cfg->vocab_size < 2 || cfg->dim < 1 || cfg->dim > 512 || cfg->window < 1 || cfg->window > 31 ||
         cfg->negative < 0 || cfg->negative > 64 || cfg->batch_size < 1 || cfg->batch_size > 64 ||
         (cfg->precision != PAR_SGNS_FP32 && cfg->precision != PAR_SGNS_BF16) ||
         (cfg->rounding != PAR_SGNS_ROUND_STOCHASTIC && cfg->rounding != PAR_SGNS_ROUND_NEAREST))
@@ -465,7 +465,7 @@ static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, i
     }
 }
 
-int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_stats *stats) {
+static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_stats *stats, bool prepare_only) {
     if (!h) return PAR_SGNS_EINVAL;
     if (!a) return seterr(h, PAR_SGNS_EINVAL, "null args");
     if (!h->have_exp || !h->have_weights) return seterr(h, PAR_SGNS_ESTATE, "exp table and weights must be set first");
@@ -492,6 +492,7 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
         d.ld = h->ld; d.d = h->cfg.dim; d.W = h->cfg.window; d.B = h->cfg.batch_size; d.kmax = h->kmax;
         d.compute_loss = a->compute_loss; d.s_begin = a->sent_begin; d.s_end = a->sent_end;
         d.loss = h->loss; d.counters = h->counters;
+        if (prepare_only) return 0;
         HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, nullptr, h->st));
         HIPCHK(h, hipEventRecord(h->ev0, h->st));
         HIPCHK(h, sgns::launch_det(d, h->st));
@@ -521,7 +522,17 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
         f.w_in = h->w_in; f.w_out = h->w_out;
         f.ld = h->ld; f.d = h->cfg.dim; f.nq = h->nq; f.W = h->cfg.window; f.R = h->R; f.NEG = h->cfg.negative;
         f.B = h->cfg.batch_size; f.V = h->cfg.vocab_size; f.kmax = h->kmax;
-        f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size);
+        if (a->repulsion < PAR_SGNS_REP_AUTO || a->repulsion > PAR_SGNS_REP_MATRIX)
+            return seterr(h, PAR_SGNS_EINVAL, "repulsion must be AUTO, VECTOR or MATRIX");
+        // the matrix-core path addresses W_out with 32-bit byte offsets and
+        // holds a batch's positions in one group of 32
+        const bool small = (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) &&
+                           h->cfg.batch_size <= 16 * sgns::kMfI;
+        if (a->repulsion == PAR_SGNS_REP_MATRIX && (!h->bf16 || !small))
+            return seterr(h, PAR_SGNS_EINVAL, "matrix-core repulsion runs on bf16 tables below 4 GiB with batch size <= 32");
+        f.rep_mfma = h->bf16 && small && a->repulsion != PAR_SGNS_REP_VECTOR;
+        f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld, f.rep_mfma);
+        f.lds_region = sgns::fast_lds_region(h->R, h->nq, h->bf16, h->cfg.batch_size, h->ld, f.rep_mfma);
         f.persistent = pers ? 1 : 0; f.rounding = h->cfg.rounding; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
         f.a0 = a->alpha; f.T = a->train_words; f.seed = a->seed; f.s_begin = a->sent_begin; f.s_end = a->sent_end;
         f.queue = h->queue; f.loss = h->loss; f.counters = h->counters;
@@ -532,7 +543,7 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
         if (pers && !planned && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
         if (pers && a->use_graph) {
-            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0), wpb, blocks, a->iter, 0,
+            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0) | (f.rep_mfma ? 4 : 0), wpb, blocks, a->iter, 0,
                        a->seed, a->train_words};
             std::memcpy(&k.alpha_bits, &a->alpha, 4);
             auto it = h->graphs.find(k);
@@ -551,10 +562,12 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
                 if (e4 != hipSuccess) return seterr(h, PAR_SGNS_EDEVICE, "graph instantiate: %s", hipGetErrorString(e4));
                 it = h->graphs.emplace(k, ge).first;
             }
+            if (prepare_only) return 0;
             HIPCHK(h, hipEventRecord(h->ev0, h->st));
             HIPCHK(h, hipGraphLaunch(it->second, h->st));
             HIPCHK(h, hipEventRecord(h->ev1, h->st));
         } else {
+            if (prepare_only) return 0;
             HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, h->queue, h->st));
             HIPCHK(h, hipEventRecord(h->ev0, h->st));
             HIPCHK(h, sgns::launch_fast(f, h->bf16, h->np, wpb, blocks, h->st));
@@ -584,6 +597,12 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
     }
     return 0;
 }
+
+int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_stats *stats) {
+    return train_impl(h, a, stats, false);
+}
+
+int par_sgns_prepare(par_sgns_handle h, const par_sgns_train_args *a) { return train_impl(h, a, nullptr, true); }
 
 static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg, int64_t nw,
                        int32_t nctx, int32_t nneg, int32_t flags, double *loss_out, int adjoint) {

@@ -37,20 +37,40 @@ constexpr int kMaxRedraw = 64;
 constexpr int kUnusedSlot = -(1 << 30);
 
 // ---------------------------------------------------------------- stores
-__device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, float2 dlt, uint32_t, int) {
+// Every write-back is a no-return atomic: fp32 tables add the fp32 delta;
+// bf16 tables decide the rounding of base + delta against the wave's own
+// snapshot `base` (stochastic or nearest), and add the resulting bf16 step
+// s - base (exact in bf16) with a packed bf16 atomic.  With no concurrent
+// writer the row ends exactly at the rounded value; a concurrent update is
+// kept, never overwritten.  A step that rounds to zero issues no write.
+typedef __bf16 __attribute__((ext_vector_type(2))) bf16x2_t;
+
+__device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, float2 base, float2 dlt, uint32_t &,
+                                        int) {
     float *p = w + row * ld + 2 * q;
     if (dlt.x != 0.0f) unsafeAtomicAdd(p, dlt.x);
     if (dlt.y != 0.0f) unsafeAtomicAdd(p + 1, dlt.y);
 }
-__device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 dlt, uint32_t noise,
-                                        int nearest) {
+__device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 base, float2 dlt,
+                                        uint32_t &rs, int nearest) {
     if (dlt.x == 0.0f && dlt.y == 0.0f) return;
-    uint32_t *p = reinterpret_cast<uint32_t *>(w + row * ld) + q;
-    const uint32_t cur = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const float x = bf16lo(cur) + dlt.x, y = bf16hi(cur) + dlt.y;
-    const uint32_t out = nearest ? (bf16_rne(x) | (bf16_rne(y) << 16))
-                                 : (bf16_sr(x, noise) | (bf16_sr(y, noise >> 16) << 16));
-    __hip_atomic_store(p, out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    rs ^= rs << 13;  // per-lane xorshift32 stream (Marsaglia 2003) for rounding noise
+    rs ^= rs >> 17;
+    rs ^= rs << 5;
+    const uint32_t noise = rs;
+    const float x = base.x + dlt.x, y = base.y + dlt.y;
+    // both roundings add a value below the kept 16 bits and truncate: half an
+    // ulp (+1 on ties to even) for nearest, 16 uniform bits for stochastic
+    const uint32_t bx = __builtin_bit_cast(uint32_t, x), by = __builtin_bit_cast(uint32_t, y);
+    const uint32_t ax = nearest ? 0x7FFFu + ((bx >> 16) & 1u) : (noise & 0xFFFFu);
+    const uint32_t ay = nearest ? 0x7FFFu + ((by >> 16) & 1u) : (noise >> 16);
+    const uint32_t rx = (bx + ax) >> 16, ry = (by + ay) >> 16;
+    const float sx = __builtin_bit_cast(float, rx << 16) - base.x;
+    const float sy = __builtin_bit_cast(float, ry << 16) - base.y;
+    if (sx == 0.0f && sy == 0.0f) return;
+    const uint32_t step = bf16_rne(sx) | (bf16_rne(sy) << 16);
+    bf16x2_t *p = reinterpret_cast<bf16x2_t *>(w + row * ld) + q;
+    __builtin_amdgcn_global_atomic_fadd_v2bf16(p, __builtin_bit_cast(bf16x2_t, step));
 }
 
 template <typename T, int NP>
@@ -59,7 +79,7 @@ __device__ __forceinline__ void load_row(const T *w, int64_t row, int ld, int nq
 #pragma unroll
     for (int t = 0; t < NP; t++) {
         const int q = lane + kWave * t;
-        if (q < nq) out[t] = Pair<T>::load(w, row, ld, q);
+        if (t < NP - 1 || q < nq) out[t] = Pair<T>::load(w, row, ld, q);
         else out[t] = typename Pair<T>::P{};
     }
 }
@@ -70,18 +90,14 @@ __device__ __forceinline__ void to_f(const typename Pair<T>::P (&in)[NP], float2
     for (int t = 0; t < NP; t++) out[t] = Pair<T>::tof(in[t]);
 }
 
-__device__ __forceinline__ uint32_t noise_of(uint32_t salt, int64_t row, int q) {
-    return hash32(salt ^ hash32((uint32_t)row * 0x9E3779B1u + (uint32_t)q));
-}
 
 template <typename T, int NP>
-__device__ __forceinline__ void store_delta(T *w, int64_t row, int ld, int nq, int lane, const float2 (&cur)[NP],
-                                            const float2 (&ref)[NP], uint32_t salt, int nearest) {
+__device__ __forceinline__ void store_delta(T *w, int64_t row, int ld, int nq, int lane, const float2 (&base)[NP],
+                                            const float2 (&dlt)[NP], uint32_t &rs, int nearest) {
 #pragma unroll
     for (int t = 0; t < NP; t++) {
         const int q = lane + kWave * t;
-        if (q < nq)
-            wb_pair(w, row, ld, q, make_float2(cur[t].x - ref[t].x, cur[t].y - ref[t].y), noise_of(salt, row, q), nearest);
+        if (t < NP - 1 || q < nq) wb_pair(w, row, ld, q, base[t], dlt[t], rs, nearest);
     }
 }
 
@@ -92,18 +108,278 @@ __device__ __forceinline__ int alias_draw(const float *prob, const int32_t *alia
     return u2 < prob[bucket] ? (int)bucket : alias[bucket];
 }
 
+// ------------------------------------------- repulsion on the matrix cores
+// One batch of the repulsion for bf16 tables.  C = the batch's W_in rows
+// (snapshot in LDS, zero rows up to a multiple of 16, zero columns up to
+// ld), N = the W_out rows of the unique negatives, m = their multiplicities:
+//   G    = N C^T                v_mfma_f32_16x16x32_bf16 (bf16 operands are
+//                               the table values, products exact, fp32 sums)
+//   Gm   = m (0 - sigmoid(G)) alpha   (exp-table sigmoid, cpu:641-645), bf16
+//   dC^T = N^T Gm               v_mfma_f32_16x16x16_bf16
+//   dN^T = C^T Gm^T             v_mfma_f32_16x16x16_bf16
+// the three cblas_sgemm calls of cpu:633-656, from one snapshot.  An output
+// tile holds four consecutive dimensions of one row per lane (D layout of a
+// 16x16 MFMA: lane l, register r -> row 4(l/16)+r, column l%16), written back
+// as two packed bf16 pairs: each row of C takes one write-back per group of
+// kMfU*16 unique negatives (one per batch up to 80), each row of N one.
+// The batch holds at most kMfI*16 = 32 positions (the host keeps B <= 32 on
+// this path), so every product reads the one snapshot the reference uses.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short s16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+
+// W_out reads of the matrix-core path go through a buffer resource: 32-bit
+// byte offsets (the host enables the path only for tables below 4 GiB) and
+// the agent-coherent cache policy (sc1) of the row loads.
+constexpr int kSc1 = 16;
+constexpr int kRsrcWord3 = 0x00020000;
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+
+template <int NP>
+__device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t *bat, int isz, int U,
+                                               const int32_t *uneg, const int32_t *umul, char *lds, int lane,
+                                               float alpha_r, double &loss_r, uint32_t &rs, const float *et) {
+    uint16_t *w_in = static_cast<uint16_t *>(a.w_in);
+    uint16_t *w_out = static_cast<uint16_t *>(a.w_out);
+    const int ld = a.ld, d = a.d, SC = mf_snap_stride(ld);
+    const uint32_t row_bytes = (uint32_t)ld * 2u;
+    const __amdgpu_buffer_rsrc_t rout =
+        __builtin_amdgcn_make_buffer_rsrc(w_out, (short)0, (int)((uint32_t)a.V * row_bytes), kRsrcWord3);
+    uint16_t *snapC = reinterpret_cast<uint16_t *>(lds);
+    uint16_t *tr = snapC + kMfI * 16 * SC;
+    const int r16 = lane & 15, g4 = lane >> 4;
+    const int KS = ld / 32, JT = (d + 15) / 16;
+    const int ni = isz, IT = (ni + 15) >> 4;
+    double lr = 0.0;  // this lane's share of the batch loss (summed over the wave at the end)
+    {
+        // snapshot of the batch rows (after this sentence's attraction), eight in flight
+        for (int i = 0; i < IT * 16; i += 8) {
+            uint32_t rows[8][NP];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (i + k < ni) load_row<uint16_t, NP>(w_in, bat[i + k], ld, a.nq, lane, rows[k]);
+                else {
+#pragma unroll
+                    for (int t = 0; t < NP; t++) rows[k][t] = 0u;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+#pragma unroll
+                for (int t = 0; t < NP; t++) {
+                    const int q = lane + kWave * t;
+                    if (2 * q < ld) *reinterpret_cast<uint32_t *>(snapC + (i + k) * SC + 2 * q) = rows[k][t];
+                }
+            }
+        }
+        int32_t rowC[kMfI];
+#pragma unroll
+        for (int it = 0; it < kMfI; it++) rowC[it] = it * 16 + r16 < ni ? bat[it * 16 + r16] : -1;
+
+        for (int u0 = 0; u0 < U; u0 += 16 * kMfU) {
+            const int nu = U - u0 < 16 * kMfU ? U - u0 : 16 * kMfU;
+            const int UT = (nu + 15) >> 4;
+            s16x4_t gd[kMfU][kMfI];
+            int32_t rowN[kMfU];
+            // ---- G = N C^T and the gradient Gm (bf16, D layout)
+#pragma unroll
+            for (int ut = 0; ut < kMfU; ut++) {
+                rowN[ut] = -1;
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) gd[ut][it] = s16x4_t{0, 0, 0, 0};
+                if (ut >= UT) continue;
+                const int un = u0 + ut * 16 + r16;
+                rowN[ut] = un < U ? uneg[un] : -1;
+                const uint32_t offN = rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes + 16u * (uint32_t)g4 : kNoRow;
+                // this tile's 16 rows of N, all of ld: 16 bytes per lane per 32 dimensions
+                u32x4_t nv[4 * NP];
+#pragma unroll
+                for (int ks = 0; ks < 4 * NP; ks++) {
+                    nv[ks] = u32x4_t{0u, 0u, 0u, 0u};
+                    if (ks < KS && offN != kNoRow) nv[ks] = __builtin_amdgcn_raw_buffer_load_b128(rout, offN + 64u * ks, 0, kSc1);
+                }
+                f32x4_t acc[kMfI];
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) acc[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < 4 * NP; ks++) {
+                    if (ks >= KS) continue;
+                    const bf16x8_t av = __builtin_bit_cast(bf16x8_t, nv[ks]);
+#pragma unroll
+                    for (int it = 0; it < kMfI; it++) {
+                        if (it >= IT) continue;
+                        const uint16_t *q = snapC + (it * 16 + r16) * SC + ks * 32 + 8 * g4;
+                        const uint2 b0 = *reinterpret_cast<const uint2 *>(q);
+                        const uint2 b1 = *reinterpret_cast<const uint2 *>(q + 4);
+                        const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, u32x4_t{b0.x, b0.y, b1.x, b1.y});
+                        acc[it] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[it], 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) {
+                    if (it >= IT) continue;
+                    uint32_t gb[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int ul = u0 + ut * 16 + 4 * g4 + r;  // D row: negative
+                        const int il = it * 16 + r16;              // D column: position
+                        float gm = 0.0f;
+                        if (ul < U && il < ni) {
+                            const float f = acc[it][r];
+                            const float m = (float)umul[ul];
+                            gm = grad_label0(f, alpha_r, et) * m;
+                            if (a.compute_loss) lr += (double)m * (double)softplusf(f);
+                        }
+                        gb[r] = bf16_rne(gm);
+                    }
+                    gd[ut][it] = s16x4_t{(short)gb[0], (short)gb[1], (short)gb[2], (short)gb[3]};
+                }
+                // keep the next tile's row prefetch (4*NP x 16 bytes per lane) out of this one
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            // ---- dC^T = N^T Gm, sixteen dimensions at a time (N as read for G:
+            // no W_out row of the group has been written yet).  Vector memory
+            // operations of a wave retire in issue order, so the A operands of
+            // step jt+1 are loaded before step jt's write-back atomics: waiting
+            // for them then never waits for those atomics.
+            auto load_a = [&](int jt, s16x4_t (&av)[kMfU]) {
+                const uint32_t jb = 2u * (uint32_t)(jt * 16 + r16);
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int ua = u0 + ut * 16 + 4 * g4 + r;
+                        av[ut][r] = 0;
+                        if (ut < UT && ua < U)
+                            av[ut][r] = (short)__builtin_amdgcn_raw_buffer_load_b16(
+                                rout, (uint32_t)uneg[ua] * row_bytes + jb, 0, kSc1);
+                    }
+                }
+            };
+            s16x4_t an[kMfU];
+            load_a(0, an);
+            for (int jt = 0; jt < JT; jt++) {
+                const int jD = jt * 16 + 4 * g4;  // first of this lane's four output dimensions
+                s16x4_t ac[kMfU];
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) ac[ut] = an[ut];
+                if (jt + 1 < JT) load_a(jt + 1, an);
+                f32x4_t accC[kMfI];
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) accC[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) {
+                    if (ut >= UT) continue;
+#pragma unroll
+                    for (int it = 0; it < kMfI; it++) {
+                        if (it >= IT) continue;
+                        accC[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ac[ut], gd[ut][it], accC[it], 0, 0, 0);
+                    }
+                }
+                if (jD >= d) continue;
+                const bool two = jD + 2 < d;  // d is even: a pair is wholly inside or outside
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) {
+                    if (it >= IT || rowC[it] < 0) continue;
+                    const uint2 b = *reinterpret_cast<const uint2 *>(snapC + (it * 16 + r16) * SC + jD);
+                    wb_pair(w_in, rowC[it], ld, jD >> 1, Pair<uint16_t>::tof(b.x), make_float2(accC[it][0], accC[it][1]),
+                            rs, a.rounding);
+                    if (two)
+                        wb_pair(w_in, rowC[it], ld, (jD >> 1) + 1, Pair<uint16_t>::tof(b.y),
+                                make_float2(accC[it][2], accC[it][3]), rs, a.rounding);
+                }
+            }
+            // ---- Gm^T: each 16 x 16 tile transposed through LDS
+            s16x4_t gt[kMfU][kMfI];
+#pragma unroll
+            for (int ut = 0; ut < kMfU; ut++) {
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) {
+                    gt[ut][it] = s16x4_t{0, 0, 0, 0};
+                    if (ut >= UT || it >= IT) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) tr[(4 * g4 + r) * kMfTS + r16] = (uint16_t)gd[ut][it][r];
+                    const uint2 t2 = *reinterpret_cast<const uint2 *>(tr + r16 * kMfTS + 4 * g4);
+                    gt[ut][it] = __builtin_bit_cast(s16x4_t, t2);
+                }
+            }
+            // ---- dN^T = C^T Gm^T, sixteen dimensions at a time; the current
+            // values the write-backs round against are fetched one step ahead
+            auto load_base = [&](int jt, u32x2_t (&bv)[kMfU]) {
+                const int jD = jt * 16 + 4 * g4;
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) {
+                    bv[ut] = u32x2_t{0u, 0u};
+                    if (ut < UT && rowN[ut] >= 0 && jD < d)
+                        bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, (uint32_t)rowN[ut] * row_bytes + 2u * (uint32_t)jD,
+                                                                      0, kSc1);
+                }
+            };
+            u32x2_t bnext[kMfU];
+            load_base(0, bnext);
+            for (int jt = 0; jt < JT; jt++) {
+                const int jA = jt * 16 + r16;
+                const int jD = jt * 16 + 4 * g4;
+                u32x2_t bcur[kMfU];
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) bcur[ut] = bnext[ut];
+                if (jt + 1 < JT) load_base(jt + 1, bnext);
+                f32x4_t accN[kMfU];
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) accN[ut] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) {
+                    if (it >= IT) continue;
+                    s16x4_t cv;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) cv[r] = (short)snapC[(it * 16 + 4 * g4 + r) * SC + jA];
+#pragma unroll
+                    for (int ut = 0; ut < kMfU; ut++) {
+                        if (ut >= UT) continue;
+                        accN[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cv, gt[ut][it], accN[ut], 0, 0, 0);
+                    }
+                }
+                if (jD >= d) continue;
+                const bool two = jD + 2 < d;
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) {
+                    if (ut >= UT || rowN[ut] < 0) continue;
+                    const u32x2_t b = bcur[ut];
+                    wb_pair(w_out, rowN[ut], ld, jD >> 1, Pair<uint16_t>::tof(b.x), make_float2(accN[ut][0], accN[ut][1]),
+                            rs, a.rounding);
+                    if (two)
+                        wb_pair(w_out, rowN[ut], ld, (jD >> 1) + 1, Pair<uint16_t>::tof(b.y),
+                                make_float2(accN[ut][2], accN[ut][3]), rs, a.rounding);
+                }
+            }
+        }
+    }
+    loss_r += lr;
+}
+
+// Counts of one sentence (scalars); the kernel folds them into per-lane
+// 64-bit totals so no 64-bit counter stays live in scalar registers.
+struct SentCount {
+    uint32_t tokens, pairs, batches, negatives;
+    uint64_t rep_pairs;
+};
+
 // ------------------------------------------------------ one sentence
 // `raw` is the training progress in raw words when the sentence starts: its
 // corpus offset in sentence order, or the words dequeued before it when a
 // queue reorders sentences -- the reference decays alpha with the words
 // processed so far (word_count_actual, cpu:431-445), not with corpus position.
-template <typename T, int NP, int NB>
-__device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int lane, char *lds, double &loss_a,
-                               double &loss_r, uint64_t (&cnt)[kCntSlots], const float *et) {
+template <typename T, int NP, int NB, bool MF>
+__device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int lane, char *lds, double &loss_a,
+                               double &loss_r, SentCount &cnt, const float *et) {
     using PT = typename Pair<T>::P;
     T *w_in = static_cast<T *>(a.w_in);
     T *w_out = static_cast<T *>(a.w_out);
     const int ld = a.ld, nq = a.nq, W = a.W, R = a.R;
+    // the host picks NP = ceil(nq / 64): only a lane's last pair can be past the row
+    __builtin_assume(nq > kWave * (NP - 1) && nq <= kWave * NP);
     const int64_t o0 = a.sent_off[s];
     const int L = (int)(a.sent_off[s + 1] - o0);
     if (L <= 0) return;  // an empty sentence trains nothing (cpu:539-542, 575)
@@ -112,70 +388,73 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
     const float alpha = planned ? a.plan_alpha[2 * s + 1] : sentence_alpha(a.a0, a.epoch, a.T, a.iter, raw);
     const float alpha0 = planned ? a.plan_alpha[2 * s] : alpha;  // centre 0 (cpu:431-447 timing)
     const uint32_t ep = (uint32_t)a.epoch;
-    const uint32_t salt0 = hash32((uint32_t)a.seed ^ hash32((uint32_t)s * 0x632BE5ABu ^ (ep << 24)));
+    // rounding-noise stream of this lane for this sentence (never zero)
+    uint32_t rs = hash32((uint32_t)a.seed ^ hash32((uint32_t)s * 0x632BE5ABu ^ (ep << 24)) ^ ((uint32_t)lane << 8)) | 1u;
     // LDS of this wave: ring values (fp32), ring snapshots (storage type),
-    // shared negatives, and the slot tables of the ring
+    // shared negatives
     float2 *ring_u = reinterpret_cast<float2 *>(lds);
     PT *ring_0 = reinterpret_cast<PT *>(lds + (size_t)R * nq * sizeof(float2));
-    const int ring_bytes = R * nq * (int)(sizeof(float2) + sizeof(PT));
-    const int snap_bytes = a.B * nq * (int)sizeof(PT);
-    int32_t *negs = reinterpret_cast<int32_t *>(lds + (((ring_bytes > snap_bytes ? ring_bytes : snap_bytes) + 15) & ~15));
-    int32_t *slot_word = negs + a.kmax;  // word held by a slot
-    int32_t *slot_last = slot_word + R;  // last window position referencing it
-    int32_t *pos_slot = slot_last + R;   // slot of position c, indexed c % R
+    int32_t *negs = reinterpret_cast<int32_t *>(lds + a.lds_region);
+    // Slot tables of the ring, one entry per lane (R = 2W+1 <= 64): lane k
+    // holds the word of slot k, the last window position referencing slot k,
+    // and the slot of the positions c with (c & pmask) == k.
+    const int pmask = pow2_ceil(R) - 1;
+    int slot_word = -1, slot_last = kUnusedSlot, pos_slot = 0;
+    // all ones if this lane's last pair lies inside the row
+    // (kept opaque so it stays a per-lane bit mask, not a scalar lane predicate)
+    uint32_t last_mask = lane + kWave * (NP - 1) < nq ? 0xFFFFFFFFu : 0u;
+    asm volatile("" : "+v"(last_mask));
+    // byte stride of a ring slot, in a vector register: the pair loop's
+    // address is then one multiply-add with no scalar reload of nq
+    uint32_t slot_bytes = (uint32_t)nq * (uint32_t)sizeof(float2);
+    asm volatile("" : "+v"(slot_bytes));
+    char *ring_lane = reinterpret_cast<char *>(ring_u + lane);
 
-    cnt[kCntSentences]++;
-    cnt[kCntTokens] += (uint64_t)L;
+    cnt.tokens = (uint32_t)L;
 
     // Slot of word w among slots still referenced at or after position `from`
-    // (-1 if none).  One lane per slot, first match wins.
+    // (-1 if none), first match wins.
     auto find_slot = [&](int w, int from) -> int {
-        for (int r0 = 0; r0 < R; r0 += kWave) {
-            const int sx = r0 + lane;
-            const bool m = sx < R && slot_last[sx] >= from && slot_word[sx] == w;
-            const uint64_t mask = __builtin_amdgcn_ballot_w64(m);
-            if (mask) return r0 + (int)__builtin_ctzll(mask);
-        }
-        return -1;
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last >= from && slot_word == w);
+        return mask ? (int)__builtin_ctzll(mask) : -1;
     };
-    // A slot referenced by no position at or after `from`.
+    // A slot referenced by no position at or after `from` (the window
+    // references at most R slots, so one exists).
     auto find_free = [&](int from) -> int {
-        for (int r0 = 0; r0 < R; r0 += kWave) {
-            const int sx = r0 + lane;
-            const uint64_t mask = __builtin_amdgcn_ballot_w64(sx < R && slot_last[sx] < from);
-            if (mask) return r0 + (int)__builtin_ctzll(mask);
-        }
-        return 0;  // unreachable: the window references at most R slots
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last < from);
+        return (int)__builtin_ctzll(mask);
     };
+    auto slot_of = [&](int c) -> int { return __builtin_amdgcn_readlane(pos_slot, c & pmask); };
+    auto last_of = [&](int slot) -> int { return __builtin_amdgcn_readlane(slot_last, slot); };
+    auto word_of = [&](int slot) -> int { return __builtin_amdgcn_readlane(slot_word, slot); };
     auto fill = [&](int slot, const PT (&row)[NP]) {
 #pragma unroll
         for (int t = 0; t < NP; t++) {
             const int q = lane + kWave * t;
-            if (q < nq) {
+            if (t < NP - 1 || q < nq) {
                 ring_0[slot * nq + q] = row[t];
                 ring_u[slot * nq + q] = Pair<T>::tof(row[t]);
             }
         }
     };
     auto flush = [&](int slot) {
-        const int64_t row = slot_word[slot];
-        const uint32_t salt = salt0 ^ hash32((uint32_t)slot_last[slot] + 0x1000u);
+        const int64_t row = word_of(slot);
 #pragma unroll
         for (int t = 0; t < NP; t++) {
             const int q = lane + kWave * t;
-            if (q < nq) {
+            if (t < NP - 1 || q < nq) {
                 const float2 u = ring_u[slot * nq + q];
                 const float2 u0 = Pair<T>::tof(ring_0[slot * nq + q]);
-                wb_pair(w_in, row, ld, q, make_float2(u.x - u0.x, u.y - u0.y), noise_of(salt, row, q), a.rounding);
+                wb_pair(w_in, row, ld, q, u0, make_float2(u.x - u0.x, u.y - u0.y), rs, a.rounding);
             }
         }
     };
     auto bind = [&](int c, int slot, int w) {
-        if (lane == 0) {
-            slot_word[slot] = w;
-            slot_last[slot] = c;
-            pos_slot[c % R] = slot;
+        if (lane == slot) {
+            slot_word = w;
+            slot_last = c;
         }
+        if (lane == (c & pmask)) pos_slot = slot;
     };
 
     // ---- attraction
@@ -186,20 +465,32 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
     // window; the next centre's W_out row and entering W_in row are prefetched
     // one centre ahead.
     {
-        // unused slots sit below every window start (window starts go down to -W)
-        for (int r0 = 0; r0 < R; r0 += kWave)
-            if (r0 + lane < R) slot_last[r0 + lane] = kUnusedSlot;
+        // unused slots (slot_last = kUnusedSlot) sit below every window start
+        // (window starts go down to -W)
         const int first = W < L - 1 ? W : L - 1;
-        for (int c = 0; c <= first; c++) {
+        for (int c = 0; c <= first; c++) {  // bind the first window's positions to slots
             const int w = sen[c];
             int sl = find_slot(w, 0);
-            if (sl < 0) {
-                sl = find_free(0);
-                PT row[NP];
-                load_row<T, NP>(w_in, w, ld, nq, lane, row);
-                fill(sl, row);
-            }
+            if (sl < 0) sl = find_free(0);
             bind(c, sl, w);
+        }
+        {  // then fetch every bound slot, four rows in flight
+            uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last >= 0);
+            while (mask) {
+                int sl4[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    sl4[k] = mask ? (int)__builtin_ctzll(mask) : -1;
+                    mask &= mask ? mask - 1 : 0;
+                }
+                PT rows[4][NP];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (sl4[k] >= 0) load_row<T, NP>(w_in, word_of(sl4[k]), ld, nq, lane, rows[k]);
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (sl4[k] >= 0) fill(sl4[k], rows[k]);
+            }
         }
         PT vnext[NP];
         load_row<T, NP>(w_out, sen[0], ld, nq, lane, vnext);
@@ -224,51 +515,76 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
             PT pf[NP];
             if (p + 1 < L && !carry_next) load_row<T, NP>(w_out, sen[p + 1], ld, nq, lane, vnext);
             if (np < L && np_slot < 0) load_row<T, NP>(w_in, sen[np], ld, nq, lane, pf);
+            float fbuf = 0.0f;  // lane k: logit of the k-th pair (loss only)
+            int npair = 0;
             for (int i = b; i < 2 * W + 1 - b; i++) {
                 if (i == W) continue;
                 const int c = p - W + i;
                 if (c < 0) continue;
                 if (c >= L) break;
-                const int slot = pos_slot[c % R];
-                float2 u[NP];
+                const int slot = slot_of(c);
+                float2 *ru = reinterpret_cast<float2 *>(ring_lane + (uint32_t)slot * slot_bytes);
+                // Ring accesses without lane predicates: a lane past the row
+                // (its last pair only) reads the next entries of the ring, uses
+                // them as zero (integer mask, whatever the bits) and writes the
+                // bits it read back unchanged.  The ring region has slack for
+                // the last slot.
+                float2 raw[NP], u[NP];
                 float part = 0.0f;
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    u[t] = q < nq ? ring_u[slot * nq + q] : make_float2(0.0f, 0.0f);
+                    raw[t] = ru[kWave * t];
+                    u[t] = raw[t];
+                    if (t == NP - 1) {
+                        u[t].x = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, raw[t].x) & last_mask);
+                        u[t].y = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, raw[t].y) & last_mask);
+                    }
                     part += u[t].x * v[t].x + u[t].y * v[t].y;
                 }
                 const float f = wave_sum(part);
                 const float g = grad_label1(f, alpha_p, et);
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    if (q < nq) ring_u[slot * nq + q] = make_float2(u[t].x + g * v[t].x, u[t].y + g * v[t].y);
+                    float2 nw = make_float2(u[t].x + g * v[t].x, u[t].y + g * v[t].y);
+                    if (t == NP - 1) {
+                        const uint32_t m = last_mask;
+                        nw.x = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, nw.x) & m) |
+                                                             (__builtin_bit_cast(uint32_t, raw[t].x) & ~m));
+                        nw.y = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, nw.y) & m) |
+                                                             (__builtin_bit_cast(uint32_t, raw[t].y) & ~m));
+                    }
+                    ru[kWave * t] = nw;
                     v[t].x += g * u[t].x;
                     v[t].y += g * u[t].y;
                 }
-                cnt[kCntPairs]++;
-                if (a.compute_loss) loss_a += (double)softplusf(-f);
+                // logit of this centre's npair-th pair, kept in lane npair
+                fbuf = lane == npair ? f : fbuf;
+                npair++;
             }
-            if (!carry_next)
-                store_delta<T, NP>(w_out, sen[p], ld, nq, lane, v, v0, salt0 ^ hash32((uint32_t)p), a.rounding);
+            cnt.pairs += (uint32_t)npair;
+            if (a.compute_loss && lane < npair) loss_a += (double)softplusf(-fbuf);  // this lane's share
+            if (!carry_next) {
+                float2 dv[NP];
+#pragma unroll
+                for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
+                store_delta<T, NP>(w_out, sen[p], ld, nq, lane, v0, dv, rs, a.rounding);
+            }
             carried = carry_next;
             // position p-W leaves the window; position np enters it
             const int r = p - W;
-            const int r_slot = r >= 0 ? pos_slot[r % R] : -1;
+            const int r_slot = r >= 0 ? slot_of(r) : -1;
             if (np < L && np_slot >= 0) bind(np, np_slot, sen[np]);
-            if (r_slot >= 0 && slot_last[r_slot] == r) flush(r_slot);
+            if (r_slot >= 0 && last_of(r_slot) == r) flush(r_slot);
             if (np < L && np_slot < 0) {
                 const int sl = find_free(p + 1 - W);
                 fill(sl, pf);
                 bind(np, sl, sen[np]);
             }
         }
-        for (int r0 = 0; r0 < R; r0 += kWave) {  // slots still referenced by the last window
-            const int sx = r0 + lane;
-            uint64_t mask = __builtin_amdgcn_ballot_w64(sx < R && slot_last[sx] >= L - W);
+        {  // slots still referenced by the last window
+            uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last >= L - W);
             while (mask) {
-                const int sl = r0 + (int)__builtin_ctzll(mask);
+                const int sl = (int)__builtin_ctzll(mask);
                 mask &= mask - 1;
                 flush(sl);
             }
@@ -287,7 +603,7 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
     const int nbat = (L + B - 1) / B;
     const float alpha_r = L == 1 ? alpha0 : alpha;
     PT *snap = reinterpret_cast<PT *>(lds);
-    int32_t *uneg = pos_slot + R;   // unique negatives of the batch
+    int32_t *uneg = negs + a.kmax;  // unique negatives of the batch
     int32_t *umul = uneg + a.kmax;  // their multiplicities
     for (int bi = 0; bi < nbat; bi++) {
         const int lo = bi * B;
@@ -304,9 +620,9 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
                 (lo <= W - 1 || (L - 1) - lo <= W - 1) ? (int)(r % (uint32_t)W) : (int)(r % (uint32_t)(2 * W));
             K = rep_b * NEG;
         }
-        cnt[kCntBatches]++;
-        cnt[kCntNegatives] += (uint64_t)K;
-        cnt[kCntRepPairs] += (uint64_t)K * (uint64_t)isz;
+        cnt.batches++;
+        cnt.negatives += (uint32_t)K;
+        cnt.rep_pairs += (uint64_t)K * (uint64_t)isz;
         if (K == 0) continue;
         // draw K negatives, 64 at a time, rejecting in-sentence words (cpu:604-622)
         for (int k0 = 0; k0 < K; k0 += kWave) {
@@ -358,17 +674,41 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
             }
             U += __builtin_popcountll(fm);
         }
-        // snapshot of the batch rows (after this sentence's attraction)
-        for (int i = 0; i < isz; i++) {
-            PT row[NP];
-            load_row<T, NP>(w_in, sen[lo + i], ld, nq, lane, row);
+        if constexpr (MF) {  // matrix-core path (bf16 tables, its own kernel instance)
+            repulsion_mfma<NP>(a, sen + lo, isz, U, uneg, umul, lds, lane, alpha_r, loss_r, rs, et);
+            continue;
+        }
+        // snapshot of the batch rows (after this sentence's attraction), four in flight
+        for (int i0 = 0; i0 < isz; i0 += 4) {
+            PT rows[4][NP];
 #pragma unroll
-            for (int t = 0; t < NP; t++) {
-                const int q = lane + kWave * t;
-                if (q < nq) snap[i * nq + q] = row[t];
+            for (int k = 0; k < 4; k++)
+                if (i0 + k < isz) load_row<T, NP>(w_in, sen[lo + i0 + k], ld, nq, lane, rows[k]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (i0 + k >= isz) continue;
+#pragma unroll
+                for (int t = 0; t < NP; t++) {
+                    const int q = lane + kWave * t;
+                    if (t < NP - 1 || q < nq) snap[(i0 + k) * nq + q] = rows[k][t];
+                }
             }
         }
-        const uint32_t salt_b = salt0 ^ hash32((uint32_t)bi * 0x27D4EB2Fu + 0x5000u);
+        // W_out rows of a register block; the next block's rows are loaded
+        // before this block's write-back atomics (vector memory operations
+        // retire in issue order, so waiting for them then skips the atomics)
+        PT pre[NB][NP];
+        auto load_block = [&](int kb) {
+#pragma unroll
+            for (int k = 0; k < NB; k++) {
+                if (kb + k < U) load_row<T, NP>(w_out, uneg[kb + k], ld, nq, lane, pre[k]);
+                else {
+#pragma unroll
+                    for (int t = 0; t < NP; t++) pre[k][t] = PT{};
+                }
+            }
+        };
+        load_block(0);
         for (int kb = 0; kb < U; kb += NB) {
             const int nbk = U - kb < NB ? U - kb : NB;
             int32_t nrow[NB];
@@ -378,24 +718,21 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
             for (int k = 0; k < NB; k++) {
                 nrow[k] = k < nbk ? uneg[kb + k] : -1;
                 mult[k] = k < nbk ? (float)umul[kb + k] : 0.0f;
-                PT tmp[NP];
-                if (k < nbk) load_row<T, NP>(w_out, nrow[k], ld, nq, lane, tmp);
-                else {
-#pragma unroll
-                    for (int t = 0; t < NP; t++) tmp[t] = PT{};
-                }
-                to_f<T, NP>(tmp, nv[k]);
+                to_f<T, NP>(pre[k], nv[k]);
 #pragma unroll
                 for (int t = 0; t < NP; t++) dn[k][t] = make_float2(0.0f, 0.0f);
             }
+            if (kb + NB < U) load_block(kb + NB);
+            const float mlane = lane < nbk ? (float)umul[kb + lane] : 0.0f;  // loss: lane k <-> negative k
             for (int i = 0; i < isz; i++) {
                 float2 c[NP];
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
                     const int q = lane + kWave * t;
-                    c[t] = q < nq ? Pair<T>::tof(snap[i * nq + q]) : make_float2(0.0f, 0.0f);
+                    c[t] = t < NP - 1 || q < nq ? Pair<T>::tof(snap[i * nq + q]) : make_float2(0.0f, 0.0f);
                 }
                 float g[NB];
+                float fbuf = 0.0f;  // lane k: logit against negative k (loss only)
 #pragma unroll
                 for (int k = 0; k < NB; k++) {
                     float part = 0.0f;
@@ -403,8 +740,9 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
                     for (int t = 0; t < NP; t++) part += nv[k][t].x * c[t].x + nv[k][t].y * c[t].y;
                     const float f = wave_sum(part);
                     g[k] = k < nbk ? grad_label0(f, alpha_r, et) : 0.0f;
-                    if (a.compute_loss && k < nbk) loss_r += (double)mult[k] * (double)softplusf(f);
+                    fbuf = lane == k ? f : fbuf;
                 }
+                if (a.compute_loss && lane < nbk) loss_r += (double)mlane * (double)softplusf(fbuf);
                 float2 dc[NP];
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
@@ -418,38 +756,46 @@ __device__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int la
                         dn[k][t].y += g[k] * c[t].y;
                     }
                 }
-                float2 zero[NP];
-#pragma unroll
-                for (int t = 0; t < NP; t++) zero[t] = make_float2(0.0f, 0.0f);
-                store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, dc, zero,
-                                   salt_b ^ hash32((uint32_t)(kb * 1024 + i)), a.rounding);
+                store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, c, dc, rs, a.rounding);
             }
 #pragma unroll
             for (int k = 0; k < NB; k++) {
                 if (k < nbk) {
-                    float2 zero[NP];
 #pragma unroll
                     for (int t = 0; t < NP; t++) {
-                        zero[t] = make_float2(0.0f, 0.0f);
                         dn[k][t].x *= mult[k];
                         dn[k][t].y *= mult[k];
                     }
-                    store_delta<T, NP>(w_out, nrow[k], ld, nq, lane, dn[k], zero,
-                                       salt_b ^ hash32((uint32_t)(kb + k) + 0x9000u), a.rounding);
+                    store_delta<T, NP>(w_out, nrow[k], ld, nq, lane, nv[k], dn[k], rs, a.rounding);
                 }
             }
         }
     }
 }
 
-template <typename T, int NP, int NB>
-__global__ __launch_bounds__(256) void sgns_fast_kernel(FastArgs a) {
+// At most 256 VGPRs + AGPRs, so two waves fit on a SIMD: the kernel is bound
+// by the latency of dependent row loads and needs resident waves to hide it.
+template <typename T, int NP, int NB, bool MF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sgns_fast_kernel(FastArgs a) {
+    static_assert(!MF || sizeof(T) == 2, "the matrix-core repulsion runs on bf16 tables");
     extern __shared__ __align__(16) char smem[];
+    __shared__ float et[kExpTable + 1];  // sigmoid table, one copy per workgroup
+    for (int i = threadIdx.x; i <= kExpTable; i += blockDim.x) et[i] = a.exp_table[i];
+    __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     char *lds = smem + (size_t)wave * a.lds_per_wave;
-    const float *et = a.exp_table;
     double loss_a = 0.0, loss_r = 0.0;
-    uint64_t cnt[kCntSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t total = 0;  // lane k: counter k of this wave (Counter order)
+    auto fold = [&](const SentCount &c) {
+        uint64_t v = 0;
+        v = lane == kCntTokens ? (uint64_t)c.tokens : v;
+        v = lane == kCntSentences ? (uint64_t)(c.tokens > 0) : v;
+        v = lane == kCntPairs ? (uint64_t)c.pairs : v;
+        v = lane == kCntBatches ? (uint64_t)c.batches : v;
+        v = lane == kCntNegatives ? (uint64_t)c.negatives : v;
+        v = lane == kCntRepPairs ? c.rep_pairs : v;
+        total += v;
+    };
     const int64_t n = a.s_end - a.s_begin;
     if (a.persistent) {
         for (;;) {
@@ -460,22 +806,31 @@ __global__ __launch_bounds__(256) void sgns_fast_kernel(FastArgs a) {
             const int64_t slot = a.s_begin + idx;
             const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
             const int64_t raw = a.plan_b ? 0 : (a.order ? a.queue_raw[slot] : a.sent_raw[s]);
-            train_sentence<T, NP, NB>(a, s, raw, lane, lds, loss_a, loss_r, cnt, et);
+            SentCount c{0u, 0u, 0u, 0u, 0ull};
+            train_sentence<T, NP, NB, MF>(a, s, raw, lane, lds, loss_a, loss_r, c, et);
+            fold(c);
         }
     } else {
         const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;
         const int64_t s = a.s_begin + idx;
-        if (idx < n) train_sentence<T, NP, NB>(a, s, a.plan_b ? 0 : a.sent_raw[s], lane, lds, loss_a, loss_r, cnt, et);
+        if (idx < n) {
+            SentCount c{0u, 0u, 0u, 0u, 0ull};
+            train_sentence<T, NP, NB, MF>(a, s, a.plan_b ? 0 : a.sent_raw[s], lane, lds, loss_a, loss_r, c, et);
+            fold(c);
+        }
     }
-    if (lane == 0) {
-        if (a.compute_loss) {
+    if (a.compute_loss) {  // every lane holds a share of the wave's losses
+#pragma unroll
+        for (int m = 1; m < kWave; m <<= 1) {
+            loss_a += __shfl_xor(loss_a, m, kWave);
+            loss_r += __shfl_xor(loss_r, m, kWave);
+        }
+        if (lane == 0) {
             atomicAdd(&a.loss[0], loss_a);
             atomicAdd(&a.loss[1], loss_r);
         }
-#pragma unroll
-        for (int k = 0; k < kCntRepPairs + 1; k++)
-            if (cnt[k]) atomicAdd(&a.counters[k], (unsigned long long)cnt[k]);
     }
+    if (lane <= kCntRepPairs && total) atomicAdd(&a.counters[lane], (unsigned long long)total);
 }
 
 // n draws of the sampler for sentence id `sent` (the counter layout of the
@@ -493,13 +848,14 @@ __global__ void sgns_reset_kernel(unsigned long long *counters, int n, double *l
     if (i == 0 && queue) *queue = 0;
 }
 
-template <typename T, int NP>
+template <typename T, int NP, bool MF>
 hipError_t launch_typed(const FastArgs &a, int wpb, int blocks, hipStream_t st) {
-    constexpr int NB = NP <= 1 ? 8 : (NP == 2 ? 6 : 4);
+    // negatives per register block: as many as fit without spilling at each width
+    constexpr int NB = NP <= 2 ? 8 : (NP == 3 ? (sizeof(T) == 2 ? 6 : 5) : 2);
     const int64_t n = a.s_end - a.s_begin;
     if (n <= 0) return hipSuccess;
     const size_t lds = (size_t)a.lds_per_wave * (size_t)wpb;
-    auto kern = sgns_fast_kernel<T, NP, NB>;
+    auto kern = sgns_fast_kernel<T, NP, NB, MF>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     dim3 grid;
@@ -513,18 +869,26 @@ hipError_t launch_typed(const FastArgs &a, int wpb, int blocks, hipStream_t st) 
 
 hipError_t launch_fast(const FastArgs &a, int bf16, int np, int wpb, int blocks, hipStream_t st) {
     if (bf16) {
+        if (a.rep_mfma) {
+            switch (np) {
+            case 1: return launch_typed<uint16_t, 1, true>(a, wpb, blocks, st);
+            case 2: return launch_typed<uint16_t, 2, true>(a, wpb, blocks, st);
+            case 3: return launch_typed<uint16_t, 3, true>(a, wpb, blocks, st);
+            case 4: return launch_typed<uint16_t, 4, true>(a, wpb, blocks, st);
+            }
+        }
         switch (np) {
-        case 1: return launch_typed<uint16_t, 1>(a, wpb, blocks, st);
-        case 2: return launch_typed<uint16_t, 2>(a, wpb, blocks, st);
-        case 3: return launch_typed<uint16_t, 3>(a, wpb, blocks, st);
-        case 4: return launch_typed<uint16_t, 4>(a, wpb, blocks, st);
+        case 1: return launch_typed<uint16_t, 1, false>(a, wpb, blocks, st);
+        case 2: return launch_typed<uint16_t, 2, false>(a, wpb, blocks, st);
+        case 3: return launch_typed<uint16_t, 3, false>(a, wpb, blocks, st);
+        case 4: return launch_typed<uint16_t, 4, false>(a, wpb, blocks, st);
         }
     } else {
         switch (np) {
-        case 1: return launch_typed<float, 1>(a, wpb, blocks, st);
-        case 2: return launch_typed<float, 2>(a, wpb, blocks, st);
-        case 3: return launch_typed<float, 3>(a, wpb, blocks, st);
-        case 4: return launch_typed<float, 4>(a, wpb, blocks, st);
+        case 1: return launch_typed<float, 1, false>(a, wpb, blocks, st);
+        case 2: return launch_typed<float, 2, false>(a, wpb, blocks, st);
+        case 3: return launch_typed<float, 3, false>(a, wpb, blocks, st);
+        case 4: return launch_typed<float, 4, false>(a, wpb, blocks, st);
         }
     }
     return hipErrorInvalidValue;

@@ -39,8 +39,10 @@ struct FastArgs {
     void *w_in, *w_out;
     int32_t ld, d, nq, W, R, NEG, B, V, kmax;
     int32_t lds_per_wave;         // bytes
+    int32_t lds_region;           // bytes of the ring / snapshot region at the start of a wave's LDS
     int32_t persistent;
     int32_t rounding;             // bf16 write-back: 0 stochastic, 1 nearest
+    int32_t rep_mfma;             // bf16 tables: repulsion products on the matrix cores
     int32_t compute_loss;
     int32_t iter, epoch;
     float a0;
@@ -79,13 +81,43 @@ hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int
 // ring values + snapshots during attraction, reused as the batch-row
 // snapshot during repulsion; then negatives (drawn, unique, multiplicity)
 // and the three ring slot tables
-inline int fast_lds_per_wave(int R, int nq, int bf16, int kmax, int B) {
+// Smallest power of two >= r (r >= 1): the position->slot table is indexed
+// by position & (pow2_ceil(R) - 1).
+__host__ __device__ inline int pow2_ceil(int r) {
+    int p = 1;
+    while (p < r) p <<= 1;
+    return p;
+}
+
+// Matrix-core repulsion (bf16 tables): a batch is processed in groups of
+// kMfI*16 positions and kMfU*16 unique negatives.  Its LDS holds the batch
+// snapshot (kMfI*16 rows of ld + kMfPad bf16; the pad spreads the four row
+// groups of a transposed read over distinct banks) and one 16 x 16 bf16 tile
+// for transposing the gradient (rows of kMfTS).
+constexpr int kMfU = 5;
+constexpr int kMfI = 2;
+constexpr int kMfPad = 4;
+constexpr int kMfTS = 20;
+__host__ __device__ inline int mf_snap_stride(int ld) { return ld + kMfPad; }
+inline int mf_bytes(int ld) { return kMfI * 16 * mf_snap_stride(ld) * 2 + 16 * kMfTS * 2; }
+
+// The region a wave's LDS starts with: the attraction ring (fp32 value +
+// loaded value per pair and slot) and, reusing it, the repulsion snapshot.
+inline int fast_lds_region(int R, int nq, int bf16, int B, int ld, int mfma) {
     const int e = bf16 ? 4 : 8;
-    int ring = R * nq * (8 + e);
+    // + one row of 64 pairs: lanes past the row read and rewrite entries
+    // beyond the last slot, which must stay inside this wave's region
+    int ring = R * nq * (8 + e) + 64 * 8;
     int snap = B * nq * e;
-    int region = ring > snap ? ring : snap;
-    region = (region + 15) & ~15;
-    return (region + 3 * kmax * 4 + 3 * R * 4 + 15) & ~15;
+    if (mfma && mf_bytes(ld) > snap) snap = mf_bytes(ld);
+    const int region = ring > snap ? ring : snap;
+    return (region + 15) & ~15;
+}
+
+inline int fast_lds_per_wave(int R, int nq, int bf16, int kmax, int B, int ld, int mfma) {
+    const int region = fast_lds_region(R, nq, bf16, B, ld, mfma);
+    (void)R;
+    return (region + 3 * kmax * 4 + 15) & ~15;
 }
 
 }  // namespace sgns

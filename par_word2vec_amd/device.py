@@ -15,6 +15,8 @@ MODES = {"det": DETERMINISTIC, "deterministic": DETERMINISTIC, "plain": PLAIN, "
 PRECISIONS = {"fp32": FP32, "bf16": BF16}
 WIN_MFMA, WIN_FP64 = 1, 2
 ROUNDINGS = {"stochastic": 0, "nearest": 1}
+# repulsion batch evaluation: matrix cores on bf16 tables unless "vector" is asked for
+REPULSIONS = {"auto": 0, "vector": 1, "matrix": 2}
 
 
 class DeviceError(RuntimeError):
@@ -107,10 +109,15 @@ class SgnsDevice:
     # ------------------------------------------------------------- training
     def train(self, mode: str | int, sent_begin: int, sent_end: int, alpha: float, train_words: int,
               iters: int = 1, epoch: int = 0, seed: int = 1, compute_loss: bool = False,
-              waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False, planned: bool = False) -> dict:
+              waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False, planned: bool = False,
+              prepare_only: bool = False, repulsion: str = "auto") -> dict:
         m = MODES[mode] if isinstance(mode, str) else int(mode)
         args = N.ParSgnsTrainArgs(m, iters, epoch, int(compute_loss), alpha, waves_per_block, blocks,
-                                  int(use_graph), int(planned), 0, train_words, seed, sent_begin, sent_end)
+                                  int(use_graph), int(planned), REPULSIONS[repulsion], train_words, seed,
+                                  sent_begin, sent_end)
+        if prepare_only:
+            self._chk(self.lib.par_sgns_prepare(self._h, C.byref(args)), "prepare")
+            return {}
         st = N.ParSgnsStats()
         self._chk(self.lib.par_sgns_train(self._h, C.byref(args), C.byref(st)), "train")
         return st.as_dict()

@@ -18,7 +18,7 @@ import pytest
 
 import oracle as O
 from par_word2vec_amd import corpora, evaluate as E, host as H
-from par_word2vec_amd.device import SgnsDevice
+from par_word2vec_amd.device import DeviceError, SgnsDevice
 from par_word2vec_amd.trainer import chunk_bounds
 
 pytestmark = pytest.mark.gpu
@@ -120,6 +120,110 @@ def test_hogwild_kernel_exact_schedule(precision_dim, window, negative, batch, s
         assert fs[k] == ds[k]
     np.testing.assert_allclose(fs["loss_attr"], ds["loss_attr"], rtol=1e-6)
     np.testing.assert_allclose(fs["loss_rep"], ds["loss_rep"], rtol=1e-6)
+
+
+def _planned_runs(vocab, stream, dim, window, negative, batch, sample, runs):
+    """Train one epoch of the oracle's exact schedule in one wavefront for each
+    (mode, precision, repulsion) in `runs`, from the same initial tables."""
+    counts, T = vocab.counts(), vocab.train_words
+    V = len(vocab)
+    plan = H.build_plan(stream, counts, T, window, negative, batch, 1, 0.025, sample)
+    # initial tables exactly representable in bf16, so every run starts equal
+    w_in0, _ = H.init_net(V, dim)
+    w_out0 = (np.random.default_rng(1).random((V, dim)).astype(np.float32) - 0.5) / dim
+    w_in0, w_out0 = ((x.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32) for x in (w_in0, w_out0))
+    out = {}
+    for mode, precision, rep in runs:
+        with SgnsDevice(V, dim, window, negative, batch, precision, rounding="nearest") as dev:
+            dev.set_exp_table(H.exp_table())
+            dev.set_weights(w_in0, w_out0)
+            dev.set_plan(plan)
+            st = dev.train(mode, 0, plan.n_sent, 0.025, T, compute_loss=True, blocks=1, waves_per_block=1,
+                           planned=mode != "det", repulsion=rep)
+            out[(mode, precision, rep)] = dev.get_weights() + (st,)
+    return out, w_in0, w_out0
+
+
+def _rms(x):
+    return float(np.sqrt(np.mean(np.square(x, dtype=np.float64))))
+
+
+@pytest.mark.parametrize("dim,window,negative,batch,single", [
+    (300, 5, 10, 24, False),
+    (100, 5, 5, 24, False),
+    (128, 8, 5, 32, False),   # the largest batch of the matrix path
+    (64, 8, 25, 24, False),   # up to 375 negatives: several negative groups
+    (300, 5, 10, 24, True),   # one-word sentences: repulsion only
+])
+def test_matrix_repulsion_exact_schedule(dim, window, negative, batch, single):
+    """bf16 tables, round-to-nearest write-backs, the oracle's exact schedule
+    in one wavefront.  The matrix-core repulsion (G on bf16 MFMA with fp32
+    sums, the gradient rounded to bf16 for dC and dN, one write-back per row
+    and batch) against the fp32 deterministic run: its error must stay at the
+    bf16 storage scale (RMS within 8% of the RMS update: round-to-nearest
+    drops updates below half an ulp, the dominant error of bf16 tables) and be
+    no larger than that of the vector repulsion on the same bf16 tables, which
+    does the same arithmetic in fp32 but rounds a row once per block of
+    negatives.  Losses are evaluated on the bf16 tables, which differ from the
+    fp32 run at that storage scale.  With one-word sentences every update of a
+    frequent negative is a write-back of its own, so W_out's storage error is
+    that of round-to-nearest alone and equal for both paths (so are the
+    losses, about 2.5% above fp32): only the relative bars apply to it."""
+    print(f"[matrix] dim={dim} W={window} neg={negative} B={batch} single={single}", flush=True)
+    if single:
+        raw, _ = H.synth_zipf(20_000, 1500, 1.0, line_len=1, seed=9)
+        vocab = H.Vocab.from_raw(raw, 1500, 5)
+        stream = H.map_raw_stream(vocab, raw)
+    else:
+        vocab, stream = _small_corpus()
+    runs = [("det", "fp32", "auto"), ("persistent", "bf16", "matrix"), ("persistent", "bf16", "vector")]
+    res, w_in0, w_out0 = _planned_runs(vocab, stream, dim, window, negative, batch, 1e-3, runs)
+    di, do, ds = res[runs[0]]
+    mi, mo, ms = res[runs[1]]
+    vi, vo, vs = res[runs[2]]
+    for k in ("pairs", "negatives", "rep_pairs", "tokens"):
+        assert ms[k] == ds[k] == vs[k]
+    for name, det, mat, vec, w0 in (("W_in", di, mi, vi, w_in0), ("W_out", do, mo, vo, w_out0)):
+        upd = _rms(det - w0)
+        em, ev = _rms(mat - det), _rms(vec - det)
+        print(f"  {name}: rms update {upd:.3e}, matrix err {em:.3e}, vector err {ev:.3e}", flush=True)
+        if not (single and name == "W_out"):
+            assert em <= 0.08 * upd
+        assert em <= ev + 1e-3 * upd
+    for k in ("loss_attr", "loss_rep"):
+        print(f"  {k}: det {ds[k]:.6e} matrix {ms[k]:.6e} vector {vs[k]:.6e}", flush=True)
+        if single:  # storage-dominated: as close to fp32 as the vector path, within 5%
+            assert abs(ms[k] - ds[k]) <= abs(vs[k] - ds[k]) + 1e-3 * abs(ds[k])
+            np.testing.assert_allclose(ms[k], ds[k], rtol=5e-2)
+        else:
+            np.testing.assert_allclose(ms[k], ds[k], rtol=5e-3)
+
+
+def test_matrix_repulsion_batch_limit():
+    """Batches above 32 positions keep one snapshot only on the vector path:
+    the matrix path is refused and AUTO runs the vector path."""
+    vocab, stream = _small_corpus()
+    runs = [("persistent", "bf16", "auto"), ("persistent", "bf16", "vector")]
+    res, _, _ = _planned_runs(vocab, stream, 64, 5, 5, 40, 1e-3, runs)
+    (ai, ao, _), (vi, vo, _) = res[runs[0]], res[runs[1]]
+    assert np.array_equal(ai, vi) and np.array_equal(ao, vo)
+    with SgnsDevice(len(vocab), 64, 5, 5, 40, "bf16") as dev:
+        dev.set_exp_table(H.exp_table())
+        dev.set_weights(*H.init_net(len(vocab), 64))
+        dev.set_plan(H.build_plan(stream, vocab.counts(), vocab.train_words, 5, 5, 40, 1, 0.025, 1e-3))
+        with pytest.raises(DeviceError, match="batch size"):
+            dev.train("persistent", 0, 10, 0.025, vocab.train_words, planned=True, repulsion="matrix")
+
+
+def test_matrix_repulsion_needs_bf16():
+    vocab, stream = _small_corpus()
+    plan = H.build_plan(stream, vocab.counts(), vocab.train_words, 5, 5, 24, 1, 0.025, 1e-3)
+    with SgnsDevice(len(vocab), 64, 5, 5, 24, "fp32") as dev:
+        dev.set_exp_table(H.exp_table())
+        dev.set_weights(*H.init_net(len(vocab), 64))
+        dev.set_plan(plan)
+        with pytest.raises(DeviceError, match="bf16"):
+            dev.train("persistent", 0, plan.n_sent, 0.025, vocab.train_words, planned=True, repulsion="matrix")
 
 
 def test_hogwild_philox_loss_curve_and_recall(zipf1m):
