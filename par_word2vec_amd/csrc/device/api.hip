@@ -526,7 +526,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
             return seterr(h, PAR_SGNS_EINVAL, "repulsion must be AUTO, VECTOR or MATRIX");
         // the matrix-core path addresses W_out with 32-bit byte offsets and
         // holds a batch's positions in one group of 32
-        const bool small = (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) &&
+        const bool small = (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) - 4096 &&
                            h->cfg.batch_size <= 16 * sgns::kMfI;
         if (a->repulsion == PAR_SGNS_REP_MATRIX && (!h->bf16 || !small))
             return seterr(h, PAR_SGNS_EINVAL, "matrix-core repulsion runs on bf16 tables below 4 GiB with batch size <= 32");

@@ -131,11 +131,45 @@ typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 
 // W_out reads of the matrix-core path go through a buffer resource: 32-bit
-// byte offsets (the host enables the path only for tables below 4 GiB) and
-// the agent-coherent cache policy (sc1) of the row loads.
+// byte offsets (the host enables the path only for tables below 4 GiB - 4 KiB)
+// and the agent-coherent cache policy (sc1) of the row loads.  A lane with no
+// row reads at kNoRow, past the resource's size: the hardware range check
+// returns zero, so every load is unconditional (a lane-predicated load whose
+// result is merged into a packed register gets a wait of its own).
 constexpr int kSc1 = 16;
 constexpr int kRsrcWord3 = 0x00020000;
-constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+constexpr uint32_t kNoRow = 0xFFFFF000u;
+
+// Write back one lane's output dimensions jD..jD+3 of row `row` (D layout of a
+// 16x16 tile: lane r of a 16-lane row holds tile row r).  Lanes r and r^8
+// swap halves (DPP row_ror:8) so that the first atomic covers tile rows 0..7
+// with 32 contiguous bytes each and the second rows 8..15: one memory request
+// per row and instruction instead of two.  Called with every lane active; a
+// lane with row < 0 or jD >= d (d even) writes nothing.
+__device__ __forceinline__ int dpp_ror8_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false); }
+__device__ __forceinline__ void wb_quad(uint16_t *w, int row, int ld, int jD, int d, uint32_t b01, uint32_t b23,
+                                        f32x4_t acc, bool lo, uint32_t &rs, int nearest) {
+    // lo lanes send their second pair, hi lanes their first
+    const float sx = lo ? acc[2] : acc[0], sy = lo ? acc[3] : acc[1];
+    const uint32_t sb = lo ? b23 : b01;
+    const float rx = __builtin_bit_cast(float, dpp_ror8_i(__builtin_bit_cast(int, sx)));
+    const float ry = __builtin_bit_cast(float, dpp_ror8_i(__builtin_bit_cast(int, sy)));
+    const uint32_t rb = (uint32_t)dpp_ror8_i((int)sb);
+    const int prow = dpp_ror8_i(row);
+    if (jD >= d) return;
+    const int q = (jD >> 1) + (lo ? 0 : 1);  // lo lanes write the first pair, hi lanes the second
+    const bool inside = lo || jD + 2 < d;
+    // rows 0..7 of the tile
+    const int ra = lo ? row : prow;
+    if (ra >= 0 && inside)
+        wb_pair(w, ra, ld, q, Pair<uint16_t>::tof(lo ? b01 : rb), lo ? make_float2(acc[0], acc[1]) : make_float2(rx, ry),
+                rs, nearest);
+    // rows 8..15
+    const int rb_row = lo ? prow : row;
+    if (rb_row >= 0 && inside)
+        wb_pair(w, rb_row, ld, q, Pair<uint16_t>::tof(lo ? rb : b23), lo ? make_float2(rx, ry) : make_float2(acc[2], acc[3]),
+                rs, nearest);
+}
 
 template <int NP>
 __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t *bat, int isz, int U,
@@ -192,13 +226,13 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 if (ut >= UT) continue;
                 const int un = u0 + ut * 16 + r16;
                 rowN[ut] = un < U ? uneg[un] : -1;
-                const uint32_t offN = rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes + 16u * (uint32_t)g4 : kNoRow;
+                const uint32_t offN = (rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes : kNoRow) + 16u * (uint32_t)g4;
                 // this tile's 16 rows of N, all of ld: 16 bytes per lane per 32 dimensions
                 u32x4_t nv[4 * NP];
 #pragma unroll
                 for (int ks = 0; ks < 4 * NP; ks++) {
                     nv[ks] = u32x4_t{0u, 0u, 0u, 0u};
-                    if (ks < KS && offN != kNoRow) nv[ks] = __builtin_amdgcn_raw_buffer_load_b128(rout, offN + 64u * ks, 0, kSc1);
+                    if (ks < KS) nv[ks] = __builtin_amdgcn_raw_buffer_load_b128(rout, offN + 64u * ks, 0, kSc1);
                 }
                 f32x4_t acc[kMfI];
 #pragma unroll
@@ -244,28 +278,43 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
             // operations of a wave retire in issue order, so the A operands of
             // step jt+1 are loaded before step jt's write-back atomics: waiting
             // for them then never waits for those atomics.
-            auto load_a = [&](int jt, s16x4_t (&av)[kMfU]) {
+            // byte offsets of this lane's A-operand rows (kNoRow past U)
+            uint32_t offA[kMfU][4];
+#pragma unroll
+            for (int ut = 0; ut < kMfU; ut++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int ua = u0 + ut * 16 + 4 * g4 + r;
+                    offA[ut][r] = ut < UT && ua < U ? (uint32_t)uneg[ua] * row_bytes : kNoRow;
+                }
+            }
+            // raw 16-bit values, one register each: packing them at load time
+            // would make the wave wait for the loads there
+            auto load_a = [&](int jt, uint32_t (&av)[kMfU][4]) {
                 const uint32_t jb = 2u * (uint32_t)(jt * 16 + r16);
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int ua = u0 + ut * 16 + 4 * g4 + r;
-                        av[ut][r] = 0;
-                        if (ut < UT && ua < U)
-                            av[ut][r] = (short)__builtin_amdgcn_raw_buffer_load_b16(
-                                rout, (uint32_t)uneg[ua] * row_bytes + jb, 0, kSc1);
+                        av[ut][r] = 0u;
+                        if (ut < UT) av[ut][r] = __builtin_amdgcn_raw_buffer_load_b16(rout, offA[ut][r] + jb, 0, kSc1);
                     }
                 }
             };
-            s16x4_t an[kMfU];
-            load_a(0, an);
+            // two steps in flight ahead of the one being computed
+            uint32_t an0[kMfU][4], an1[kMfU][4];
+            load_a(0, an0);
+            if (1 < JT) load_a(1, an1);
             for (int jt = 0; jt < JT; jt++) {
                 const int jD = jt * 16 + 4 * g4;  // first of this lane's four output dimensions
                 s16x4_t ac[kMfU];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) ac[ut] = an[ut];
-                if (jt + 1 < JT) load_a(jt + 1, an);
+                for (int ut = 0; ut < kMfU; ut++) {
+                    ac[ut] = s16x4_t{(short)an0[ut][0], (short)an0[ut][1], (short)an0[ut][2], (short)an0[ut][3]};
+#pragma unroll
+                    for (int r = 0; r < 4; r++) an0[ut][r] = an1[ut][r];
+                }
+                if (jt + 2 < JT) load_a(jt + 2, an1);
                 f32x4_t accC[kMfI];
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) accC[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -278,17 +327,11 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                         accC[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ac[ut], gd[ut][it], accC[it], 0, 0, 0);
                     }
                 }
-                if (jD >= d) continue;
-                const bool two = jD + 2 < d;  // d is even: a pair is wholly inside or outside
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) {
-                    if (it >= IT || rowC[it] < 0) continue;
+                    if (it >= IT) continue;  // uniform
                     const uint2 b = *reinterpret_cast<const uint2 *>(snapC + (it * 16 + r16) * SC + jD);
-                    wb_pair(w_in, rowC[it], ld, jD >> 1, Pair<uint16_t>::tof(b.x), make_float2(accC[it][0], accC[it][1]),
-                            rs, a.rounding);
-                    if (two)
-                        wb_pair(w_in, rowC[it], ld, (jD >> 1) + 1, Pair<uint16_t>::tof(b.y),
-                                make_float2(accC[it][2], accC[it][3]), rs, a.rounding);
+                    wb_quad(w_in, rowC[it], ld, jD, d, b.x, b.y, accC[it], r16 < 8, rs, a.rounding);
                 }
             }
             // ---- Gm^T: each 16 x 16 tile transposed through LDS
@@ -312,9 +355,9 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) {
                     bv[ut] = u32x2_t{0u, 0u};
-                    if (ut < UT && rowN[ut] >= 0 && jD < d)
-                        bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, (uint32_t)rowN[ut] * row_bytes + 2u * (uint32_t)jD,
-                                                                      0, kSc1);
+                    if (ut >= UT) continue;  // uniform; dimensions past d read the row's zero padding
+                    const uint32_t off = rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes : kNoRow;
+                    bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, off + 2u * (uint32_t)jD, 0, kSc1);
                 }
             };
             u32x2_t bnext[kMfU];
@@ -341,17 +384,10 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                         accN[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cv, gt[ut][it], accN[ut], 0, 0, 0);
                     }
                 }
-                if (jD >= d) continue;
-                const bool two = jD + 2 < d;
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) {
-                    if (ut >= UT || rowN[ut] < 0) continue;
-                    const u32x2_t b = bcur[ut];
-                    wb_pair(w_out, rowN[ut], ld, jD >> 1, Pair<uint16_t>::tof(b.x), make_float2(accN[ut][0], accN[ut][1]),
-                            rs, a.rounding);
-                    if (two)
-                        wb_pair(w_out, rowN[ut], ld, (jD >> 1) + 1, Pair<uint16_t>::tof(b.y),
-                                make_float2(accN[ut][2], accN[ut][3]), rs, a.rounding);
+                    if (ut >= UT) continue;  // uniform
+                    wb_quad(w_out, rowN[ut], ld, jD, d, bcur[ut].x, bcur[ut].y, accN[ut], r16 < 8, rs, a.rounding);
                 }
             }
         }
