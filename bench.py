@@ -10,7 +10,8 @@ captured hipGraph for the persistent variant), inputs resident in HBM.
 Both variants train the same epoch from the same InitNet weights:
   fp32 plain      one wavefront per sentence, fp32 tables, float atomics
   bf16 persistent work-queue kernel, bf16 tables, fp32 math, stochastic
-                  rounding, hipGraph per chunk
+                  rounding, repulsion products on the matrix cores, hipGraph
+                  per chunk
 `value` is the bf16 persistent rate in Mwords/s: raw corpus words (every
 token incl. those removed by subsampling and "</s>") per second of the timed
 steps.  The planted-cluster recall@10 of both runs is reported beside it.
@@ -36,6 +37,7 @@ from par_word2vec_amd.device import SgnsDevice  # noqa: E402
 from par_word2vec_amd.trainer import chunk_bounds  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "sgns_fast_kernel_1b_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -50,7 +52,7 @@ def algorithmic_bytes(tokens: int, negatives: int, dim: int, elem: int) -> float
 
 
 def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int, steps: int, csr, seed: int,
-                profile_only: bool = False) -> dict:
+                profile_only: bool = False, repulsion: str = "auto") -> dict:
     w = c.workload
     V = len(c.vocab)
     dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision)
@@ -65,7 +67,7 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     def step(i: int, prepare_only: bool = False) -> dict:
         b, e = chunks[i % n]
         return dev.train(mode, b, e, w.alpha, c.train_words, iters=w.iters, epoch=0, seed=seed, use_graph=use_graph,
-                         prepare_only=prepare_only)
+                         prepare_only=prepare_only, repulsion=repulsion)
 
     for i in range(n):  # queue orders and graphs are built before anything is timed
         step(i, prepare_only=True)
@@ -91,7 +93,8 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     negs = sum(s["negatives"] for s in stats)
     elem = 2 if precision == "bf16" else 4
     byts = algorithmic_bytes(tokens, negs, w.dim, elem)
-    out = dict(mode=mode, precision=precision, wall_s=wall, kernel_ms=kern_ms, raw_words=raw, tokens=tokens,
+    kind = "matrix" if precision == "bf16" and repulsion != "vector" else "vector"  # kernel instance
+    out = dict(mode=mode, precision=precision, kernel=f"{precision}-{kind}", wall_s=wall, kernel_ms=kern_ms, raw_words=raw, tokens=tokens,
                negatives=negs, pairs=sum(s["pairs"] for s in stats), mwords_s=raw / wall / 1e6,
                kernel_mwords_s=raw / (kern_ms / 1e3) / 1e6, alg_bytes=byts, launches=len(stats),
                avg_launch_ms=kern_ms / max(len(stats), 1), achieved_gbs=byts / (kern_ms / 1e3) / 1e9)
@@ -102,6 +105,31 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     log(f"{mode}/{precision}: {out['mwords_s']:.1f} Mwords/s wall, {out['kernel_mwords_s']:.1f} kernel, "
         f"{out['achieved_gbs']:.0f} GB/s algorithmic")
     return out
+
+
+def calibration_copy(mib: int) -> None:
+    """One elementwise kernel that reads `mib` MiB and writes `mib` MiB (a plain
+    copy of known size; torch's copy_ may go to a DMA engine, which counters do
+    not see), so a counter run of this command can scale FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    log(f"calibration copy: {mib} MiB read and written by one elementwise kernel")
+    del src, dst
+
+
+def measured_traffic(variant: str) -> float | None:
+    """HBM bytes per launch of the training kernel from the committed counter
+    profile of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
+    calibration copy), or None."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+        return float(t["variants"][variant]["bytes_per_launch"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
 
 
 def cluster_score(c, w_in) -> float | None:
@@ -150,12 +178,16 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-score", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="run a device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               f"(the multi-GPU path is a design note, not code)", file=sys.stderr)
         return 2
 
+    if args.calibrate_copy_mb > 0:
+        calibration_copy(args.calibrate_copy_mb)
     t0 = time.perf_counter()
     c = corpora.make(args.workload, args.scale)
     w = c.workload
@@ -170,9 +202,9 @@ def main() -> int:
 
     results = {}
     for v in args.variants.split(","):
-        prec, mode = v.split("-")
+        prec, mode, *rep = v.split("-")  # precision-mode[-repulsion]
         results[v] = run_variant(c, mode, prec, chunks, raw_per_chunk, args.warmup, steps, csr, args.seed,
-                                 profile_only=args.no_score)
+                                 profile_only=args.no_score, repulsion=rep[0] if rep else "auto")
     scores = {}
     if not args.no_score:
         for v, r in results.items():
@@ -204,8 +236,10 @@ def main() -> int:
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
                      for k, r in results.items()},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"sgns_fast_kernel<{'bf16' if head['precision'] == 'bf16' else 'f32'}>",
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(head["kernel"]) if w.name == "1b" else None,
+                     "algorithmic_bytes_per_launch": round(head["alg_bytes"] / max(head["launches"], 1)),
+                     "kernel": f"sgns_fast_kernel ({head['kernel']} repulsion)",
                      "avg_launch_ms": round(head["avg_launch_ms"], 4)},
         "cpu_baseline": cpu,
     }

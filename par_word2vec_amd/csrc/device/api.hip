@@ -611,6 +611,8 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
         return seterr(h, PAR_SGNS_EINVAL, "bad window batch (nctx 1..32, nneg 0..7)");
     if (!h->have_weights) return seterr(h, PAR_SGNS_ESTATE, "weights not set");
     if ((flags & PAR_SGNS_WIN_MFMA) && !h->bf16) return seterr(h, PAR_SGNS_EINVAL, "MFMA path needs bf16 tables");
+    if ((flags & PAR_SGNS_WIN_MFMA) && (flags & PAR_SGNS_WIN_FP64))
+        return seterr(h, PAR_SGNS_EINVAL, "MFMA path accumulates in fp32 (no FP64 variant)");
     const int32_t V = h->cfg.vocab_size;
     int rc;
     if ((rc = check_ids(h, centre, nw, 0, V, "centre"))) return rc;

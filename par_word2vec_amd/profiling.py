@@ -1,0 +1,92 @@
+"""Summaries of rocprofv3 output for the training kernel (the figures the
+committed profile notes and bench.py's roofline use).
+
+    python -m par_word2vec_amd.profiling stats  <kernel_stats.csv>
+    python -m par_word2vec_amd.profiling traffic <fetch counter csv> <write counter csv> <copy MiB> <out.json>
+
+`stats` prints the per-kernel dispatch count and average duration of a
+`rocprofv3 --kernel-trace --stats --output-format csv` run.  `traffic` takes
+two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per dispatch) of a
+command that first copies `copy MiB` device to device (bench.py
+--calibrate-copy-mb), scales each counter by what it reads for that copy
+(bytes per counted KiB) and writes bytes per launch for every training-kernel
+variant."""
+from __future__ import annotations
+
+import csv
+import json
+import sys
+from collections import defaultdict
+
+COPY_KERNEL = "neg_kernel"  # torch.neg of the calibration buffer (bench.py --calibrate-copy-mb)
+
+
+def variant_of(kernel_name: str) -> str | None:
+    """'bf16-persistent' style name of a training-kernel instance, else None
+    (mode is not in the symbol; plain and persistent share an instance)."""
+    if "sgns_fast_kernel" not in kernel_name:
+        return None
+    prec = "bf16" if "<unsigned short" in kernel_name else "fp32"
+    rep = "matrix" if kernel_name.rstrip(")").rstrip().endswith("true>") or ", true>" in kernel_name else "vector"
+    return f"{prec}-{rep}"
+
+
+def kernel_stats(path: str) -> list[dict]:
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append({"name": r["Name"], "calls": int(r["Calls"]),
+                         "avg_ms": float(r["AverageNs"]) / 1e6, "total_ms": float(r["TotalDurationNs"]) / 1e6})
+    return rows
+
+
+def counter_means(path: str, counter: str) -> dict[str, tuple[int, float]]:
+    acc: dict[str, list[float]] = defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+
+
+def traffic(fetch_csv: str, write_csv: str, copy_mib: float) -> dict:
+    copy_bytes = copy_mib * (1 << 20)
+    out = {"copy_bytes": copy_bytes, "counters": {}, "variants": {}}
+    per_kernel: dict[str, dict] = defaultdict(dict)
+    for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        means = counter_means(path, name)
+        copy = [v for k, v in means.items() if COPY_KERNEL in k]
+        if not copy:
+            raise SystemExit(f"no {COPY_KERNEL} dispatch in {path}")
+        n_copy, kib_copy = max(copy, key=lambda t: t[1])  # the calibration copy is the largest
+        factor = copy_bytes / kib_copy  # bytes per counted KiB
+        out["counters"][name] = {"copy_kib": kib_copy, "bytes_per_kib": factor}
+        for k, (n, kib) in means.items():
+            v = variant_of(k)
+            if v:
+                per_kernel[v][name] = {"dispatches": n, "kib": kib, "bytes": kib * factor}
+    for v, c in per_kernel.items():
+        if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+            out["variants"][v] = {"fetch_bytes": c["FETCH_SIZE"]["bytes"], "write_bytes": c["WRITE_SIZE"]["bytes"],
+                                  "bytes_per_launch": c["FETCH_SIZE"]["bytes"] + c["WRITE_SIZE"]["bytes"],
+                                  "dispatches": c["FETCH_SIZE"]["dispatches"]}
+    return out
+
+
+def main(argv: list[str]) -> int:
+    if len(argv) >= 2 and argv[0] == "stats":
+        for r in kernel_stats(argv[1]):
+            print(f"{r['calls']:8d} {r['avg_ms']:12.4f} ms avg {r['total_ms']:12.2f} ms total  {r['name'][:110]}")
+        return 0
+    if len(argv) == 5 and argv[0] == "traffic":
+        t = traffic(argv[1], argv[2], float(argv[3]))
+        with open(argv[4], "w") as f:
+            json.dump(t, f, indent=1)
+        print(json.dumps(t, indent=1))
+        return 0
+    print(__doc__)
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -12,12 +12,15 @@ Bars:
   adjoint vs the CPU pair update  -alpha * dL/dW equals the oracle's
                                attraction update (cpu:551-565) up to the
                                sigmoid table's quantisation (1% relative)
+  matrix cores (bf16 tables)   forward within 1e-6 relative (exact bf16 products,
+                               fp32 sums); adjoint within 1e-4 of max|grad| (the
+                               logit gradient enters the products as bf16 hi + lo)
 """
 import numpy as np
 import pytest
 
 import oracle as O
-from par_word2vec_amd.device import SgnsDevice, WIN_FP64
+from par_word2vec_amd.device import DeviceError, SgnsDevice, WIN_FP64, WIN_MFMA
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +75,47 @@ def test_adjoint_matches_fp64_gradient(windows, precision):
     assert loss == pytest.approx(O.window_loss_f64(centre, ctx, neg, w_in, w_out), rel=1e-6)
     assert np.abs(gi - gi_ref).max() <= 1e-5 * np.abs(gi_ref).max()
     assert np.abs(go - go_ref).max() <= 1e-5 * np.abs(go_ref).max()
+
+
+def test_matrix_forward_and_adjoint(windows):
+    """The matrix-core window path against the FP64 oracle on the same bf16
+    tables, and its time beside the vector path."""
+    w_in, w_out, centre, ctx, neg = windows
+    w_in, w_out = bf16_round(w_in), bf16_round(w_out)
+    ref = O.window_loss_f64(centre, ctx, neg, w_in, w_out)
+    gi_ref, go_ref = O.window_grad_f64(centre, ctx, neg, w_in, w_out)
+    with SgnsDevice(V, D, 5, NNEG, 24, "bf16") as dev:
+        dev.set_weights(w_in, w_out)
+        lf = dev.loss_forward(centre, ctx, neg, flags=WIN_MFMA)
+        f_ms, _ = dev.last_kernel_ms(1)
+        dev.loss_forward(centre, ctx, neg)
+        fv_ms, _ = dev.last_kernel_ms(1)
+        la = dev.loss_adjoint(centre, ctx, neg, flags=WIN_MFMA)
+        a_ms, _ = dev.last_kernel_ms(2)
+        gi, go = dev.get_grads()
+        dev.loss_adjoint(centre, ctx, neg)
+        av_ms, _ = dev.last_kernel_ms(2)
+    # useful flops: 2*nctx*(1+nneg)*d per window forward, twice that for the two adjoint products
+    f_flop = 2.0 * NCTX * (1 + NNEG) * D * NW
+    print(f"[matrix] forward {f_ms:.3f} ms (vector {fv_ms:.3f}), adjoint {a_ms:.3f} ms (vector {av_ms:.3f}); "
+          f"useful {f_flop / f_ms / 1e9:.2f} / {3 * f_flop / a_ms / 1e9:.2f} TFLOP/s of 2500 bf16; "
+          f"max|dgi| {np.abs(gi - gi_ref).max():.3e} of {np.abs(gi_ref).max():.3e}", flush=True)
+    assert lf == pytest.approx(ref, rel=1e-6)
+    assert la == pytest.approx(ref, rel=1e-6)
+    assert np.abs(gi - gi_ref).max() <= 1e-4 * np.abs(gi_ref).max()
+    assert np.abs(go - go_ref).max() <= 1e-4 * np.abs(go_ref).max()
+
+
+def test_matrix_window_errors(windows):
+    w_in, w_out, centre, ctx, neg = windows
+    with SgnsDevice(V, D, 5, NNEG, 24, "fp32") as dev:
+        dev.set_weights(w_in, w_out)
+        with pytest.raises(DeviceError, match="bf16"):
+            dev.loss_forward(centre[:8], ctx[:8], neg[:8], flags=WIN_MFMA)
+    with SgnsDevice(V, D, 5, NNEG, 24, "bf16") as dev:
+        dev.set_weights(w_in, w_out)
+        with pytest.raises(DeviceError, match="fp32"):
+            dev.loss_forward(centre[:8], ctx[:8], neg[:8], flags=WIN_MFMA | WIN_FP64)
 
 
 def test_adjoint_finite_differences(windows):
