@@ -498,14 +498,27 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     // word occurring twice in the window shares one slot, so every pair sees
     // the updates of the pairs before it exactly as cpu:551-565 does in place.
     // A slot is written back when the last position referencing it leaves the
-    // window; the next centre's W_out row and entering W_in row are prefetched
-    // one centre ahead.
+    // window.  The W_out row of centre p+2 and the W_in row entering at p+2+W
+    // are loaded while centre p trains, except when this wave may still write
+    // that row first (the W_out row of the current centre's word; a W_in row
+    // whose word holds a slot that could be flushed): those are loaded after
+    // centre p's write-backs, so every pair sees the same values as in the
+    // reference's sequential order.
     {
+        // the sentence's tokens [tb, tb+128) in two registers, lane k holding
+        // tokens tb+k and tb+64+k; every token the centre loop needs lies there
+        int tb = 0;
+        auto tok_block = [&](int base) -> int { return base + lane < L ? sen[base + lane] : -1; };
+        int tw0 = tok_block(0), tw1 = tok_block(kWave);
+        auto tok = [&](int x) -> int {
+            const int o = x - tb;
+            return o < kWave ? __builtin_amdgcn_readlane(tw0, o) : __builtin_amdgcn_readlane(tw1, o - kWave);
+        };
         // unused slots (slot_last = kUnusedSlot) sit below every window start
         // (window starts go down to -W)
         const int first = W < L - 1 ? W : L - 1;
         for (int c = 0; c <= first; c++) {  // bind the first window's positions to slots
-            const int w = sen[c];
+            const int w = tok(c);
             int sl = find_slot(w, 0);
             if (sl < 0) sl = find_free(0);
             bind(c, sl, w);
@@ -528,29 +541,47 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                     if (sl4[k] >= 0) fill(sl4[k], rows[k]);
             }
         }
-        PT vnext[NP];
-        load_row<T, NP>(w_out, sen[0], ld, nq, lane, vnext);
+        // vq0/vq1: W_out rows of centres p and p+1; pq0/pq1: W_in rows entering
+        // at p+1+W and p+2+W (used only if the word then holds no slot)
+        PT vq0[NP], vq1[NP], pq0[NP], pq1[NP];
+        load_row<T, NP>(w_out, tok(0), ld, nq, lane, vq0);
+        if (1 < L) load_row<T, NP>(w_out, tok(1), ld, nq, lane, vq1);
+        if (1 + W < L) load_row<T, NP>(w_in, tok(1 + W), ld, nq, lane, pq0);
         float2 v[NP], v0[NP];
         bool carried = false;
         uint32_t bwin = 0;
         for (int p = 0; p < L; p++) {
-            if ((p & (kWave - 1)) == 0)
+            if ((p & (kWave - 1)) == 0) {
                 bwin = planned ? (p + lane < L ? a.plan_b[o0 + p + lane] : 0u)
                                : draw(a.seed, s, ep, kTagWindow, (uint32_t)(p + lane)).x % (uint32_t)W;
+                if (p > 0) {  // slide the token window: [p, p+128)
+                    tb = p;
+                    tw0 = tw1;
+                    tw1 = tok_block(p + kWave);
+                }
+            }
             const int b = __builtin_amdgcn_readlane((int)bwin, p & (kWave - 1));
             const float alpha_p = p == 0 ? alpha0 : alpha;
+            const int wp = tok(p);
             if (!carried) {  // a centre repeated at p-1 keeps its register row
-                to_f<T, NP>(vnext, v);
+                to_f<T, NP>(vq0, v);
 #pragma unroll
                 for (int t = 0; t < NP; t++) v0[t] = v[t];
             }
-            const bool carry_next = p + 1 < L && sen[p + 1] == sen[p];
-            // prefetch for centre p+1: its W_out row and the W_in row entering its window
+            const int wn = p + 1 < L ? tok(p + 1) : -1;
+            const bool carry_next = wn == wp;
             const int np = p + 1 + W;
-            const int np_slot = np < L ? find_slot(sen[np], p - W) : -1;
-            PT pf[NP];
-            if (p + 1 < L && !carry_next) load_row<T, NP>(w_out, sen[p + 1], ld, nq, lane, vnext);
-            if (np < L && np_slot < 0) load_row<T, NP>(w_in, sen[np], ld, nq, lane, pf);
+            const int wnp = np < L ? tok(np) : -1;
+            const int np_slot = np < L ? find_slot(wnp, p - W) : -1;
+            // loads for two centres ahead that this centre's write-backs cannot affect
+#pragma unroll
+            for (int t = 0; t < NP; t++) vq0[t] = vq1[t];
+            const int w2 = p + 2 < L ? tok(p + 2) : -1;
+            const bool v_late = w2 >= 0 && w2 == wp;  // written back at the end of this centre
+            if (w2 >= 0 && !v_late) load_row<T, NP>(w_out, w2, ld, nq, lane, vq1);
+            const int e2 = np + 1 < L ? tok(np + 1) : -1;
+            const bool p_late = e2 >= 0 && find_slot(e2, p - W) >= 0;  // its slot may be flushed first
+            if (e2 >= 0 && !p_late) load_row<T, NP>(w_in, e2, ld, nq, lane, pq1);
             float fbuf = 0.0f;  // lane k: logit of the k-th pair (loss only)
             int npair = 0;
             for (int i = b; i < 2 * W + 1 - b; i++) {
@@ -603,19 +634,24 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 float2 dv[NP];
 #pragma unroll
                 for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
-                store_delta<T, NP>(w_out, sen[p], ld, nq, lane, v0, dv, rs, a.rounding);
+                store_delta<T, NP>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
             }
             carried = carry_next;
             // position p-W leaves the window; position np enters it
             const int r = p - W;
             const int r_slot = r >= 0 ? slot_of(r) : -1;
-            if (np < L && np_slot >= 0) bind(np, np_slot, sen[np]);
+            if (np < L && np_slot >= 0) bind(np, np_slot, wnp);
             if (r_slot >= 0 && last_of(r_slot) == r) flush(r_slot);
             if (np < L && np_slot < 0) {
                 const int sl = find_free(p + 1 - W);
-                fill(sl, pf);
-                bind(np, sl, sen[np]);
+                fill(sl, pq0);
+                bind(np, sl, wnp);
             }
+#pragma unroll
+            for (int t = 0; t < NP; t++) pq0[t] = pq1[t];
+            // the loads held back until this centre's write-backs were issued
+            if (v_late) load_row<T, NP>(w_out, w2, ld, nq, lane, vq1);
+            if (p_late) load_row<T, NP>(w_in, e2, ld, nq, lane, pq0);
         }
         {  // slots still referenced by the last window
             uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last >= L - W);
