@@ -71,7 +71,7 @@ typedef struct par_sgns_train_args {
     int32_t epoch;        /* current epoch, 0-based */
     int32_t compute_loss; /* accumulate the FP64 SGNS loss */
     float alpha;          /* starting learning rate alpha0 */
-    int32_t waves_per_block; /* persistent mode; 0 = tuned default */
+    int32_t waves_per_block; /* 1..8; persistent mode: 0 = the most resident waves */
     int32_t blocks;          /* persistent mode; 0 = tuned default */
     int32_t use_graph;       /* persistent mode: replay a captured hipGraph */
     int32_t schedule;        /* plain/persistent: 0 = Philox draws over the uploaded

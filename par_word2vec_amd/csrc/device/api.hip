@@ -448,16 +448,30 @@ static int ensure_order(par_sgns_handle h, int64_t b, int64_t e) {
 // resident wave of the chip turns a pass into a few huge stale steps.
 constexpr int64_t kMinSentencesPerWave = 16;
 
+// Resident blocks per CU for a block of wpb waves: 160 KB of LDS per CU, each
+// block also holding the sigmoid table; at most kMaxWavesCu waves per CU
+// (two per SIMD at the kernel's register budget).
+constexpr int kLdsCu = 160 * 1024, kBlockStatic = 4096, kMaxWavesCu = 8, kMaxWpb = 8;
+static int blocks_per_cu(int lds_per_wave, int wpb) {
+    int b = kLdsCu / (lds_per_wave * wpb + kBlockStatic);
+    if (b * wpb > kMaxWavesCu) b = kMaxWavesCu / wpb;
+    return b;
+}
+
 static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, int &wpb, int &blocks) {
-    if (wpb <= 0) {
-        wpb = 4;
-        while (wpb > 1 && (size_t)lds_per_wave * wpb > 64 * 1024) wpb /= 2;
+    if (wpb <= 0) {  // the block size that keeps the most waves resident (smallest on ties)
+        int best = 0;
+        wpb = 1;
+        for (int k = 1; k <= kMaxWpb; k++) {
+            const int waves = blocks_per_cu(lds_per_wave, k) * k;
+            if (waves > best) {
+                best = waves;
+                wpb = k;
+            }
+        }
     }
     if (blocks <= 0) {
-        const int lds_cu = 160 * 1024;
-        int waves_cu = lds_per_wave > 0 ? lds_cu / lds_per_wave : 32;
-        if (waves_cu > 32) waves_cu = 32;
-        int bpc = waves_cu / wpb;
+        int bpc = blocks_per_cu(lds_per_wave, wpb);
         if (bpc < 1) bpc = 1;
         blocks = bpc * (h->cu_count > 0 ? h->cu_count : 256);
         const int64_t cap = n_sent / kMinSentencesPerWave / wpb;
@@ -539,6 +553,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         int wpb = pers ? a->waves_per_block : 1, blocks = a->blocks;
         default_shape(h, f.lds_per_wave, a->sent_end - a->sent_begin, wpb, blocks);
         if (!pers) wpb = a->waves_per_block > 0 ? a->waves_per_block : 1;
+        if (wpb > kMaxWpb) return seterr(h, PAR_SGNS_EINVAL, "at most %d waves per block", kMaxWpb);
         if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
         if (pers && !planned && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;

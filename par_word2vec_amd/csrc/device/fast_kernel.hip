@@ -182,14 +182,17 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
     const __amdgpu_buffer_rsrc_t rout =
         __builtin_amdgcn_make_buffer_rsrc(w_out, (short)0, (int)((uint32_t)a.V * row_bytes), kRsrcWord3);
     uint16_t *snapC = reinterpret_cast<uint16_t *>(lds);
-    uint16_t *tr = snapC + kMfI * 16 * SC;
+    uint16_t *tr = snapC + (a.B + 1) * SC;
     const int r16 = lane & 15, g4 = lane >> 4;
     const int KS = ld / 32, JT = (d + 15) / 16;
     const int ni = isz, IT = (ni + 15) >> 4;
+    auto srow = [&](int i) { return i < ni ? i : a.B; };  // tile rows past the batch read the zero row
     double lr = 0.0;  // this lane's share of the batch loss (summed over the wave at the end)
     {
-        // snapshot of the batch rows (after this sentence's attraction), eight in flight
-        for (int i = 0; i < IT * 16; i += 8) {
+        // snapshot of the batch rows (after this sentence's attraction), eight in
+        // flight, and the zero row
+        for (int q = lane; 2 * q < ld; q += kWave) *reinterpret_cast<uint32_t *>(snapC + a.B * SC + 2 * q) = 0u;
+        for (int i = 0; i < ni; i += 8) {
             uint32_t rows[8][NP];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -204,7 +207,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
                     const int q = lane + kWave * t;
-                    if (2 * q < ld) *reinterpret_cast<uint32_t *>(snapC + (i + k) * SC + 2 * q) = rows[k][t];
+                    if (i + k < ni && 2 * q < ld) *reinterpret_cast<uint32_t *>(snapC + (i + k) * SC + 2 * q) = rows[k][t];
                 }
             }
         }
@@ -244,7 +247,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                     for (int it = 0; it < kMfI; it++) {
                         if (it >= IT) continue;
-                        const uint16_t *q = snapC + (it * 16 + r16) * SC + ks * 32 + 8 * g4;
+                        const uint16_t *q = snapC + srow(it * 16 + r16) * SC + ks * 32 + 8 * g4;
                         const uint2 b0 = *reinterpret_cast<const uint2 *>(q);
                         const uint2 b1 = *reinterpret_cast<const uint2 *>(q + 4);
                         const bf16x8_t bv = __builtin_bit_cast(bf16x8_t, u32x4_t{b0.x, b0.y, b1.x, b1.y});
@@ -330,7 +333,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) {
                     if (it >= IT) continue;  // uniform
-                    const uint2 b = *reinterpret_cast<const uint2 *>(snapC + (it * 16 + r16) * SC + jD);
+                    const uint2 b = *reinterpret_cast<const uint2 *>(snapC + srow(it * 16 + r16) * SC + jD);
                     wb_quad(w_in, rowC[it], ld, jD, d, b.x, b.y, accC[it], r16 < 8, rs, a.rounding);
                 }
             }
@@ -377,7 +380,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                     if (it >= IT) continue;
                     s16x4_t cv;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) cv[r] = (short)snapC[(it * 16 + 4 * g4 + r) * SC + jA];
+                    for (int r = 0; r < 4; r++) cv[r] = (short)snapC[srow(it * 16 + 4 * g4 + r) * SC + jA];
 #pragma unroll
                     for (int ut = 0; ut < kMfU; ut++) {
                         if (ut >= UT) continue;
@@ -848,7 +851,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
 // At most 256 VGPRs + AGPRs, so two waves fit on a SIMD: the kernel is bound
 // by the latency of dependent row loads and needs resident waves to hide it.
 template <typename T, int NP, int NB, bool MF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void sgns_fast_kernel(FastArgs a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void sgns_fast_kernel(FastArgs a) {
     static_assert(!MF || sizeof(T) == 2, "the matrix-core repulsion runs on bf16 tables");
     extern __shared__ __align__(16) char smem[];
     __shared__ float et[kExpTable + 1];  // sigmoid table, one copy per workgroup

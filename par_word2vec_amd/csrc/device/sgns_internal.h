@@ -89,17 +89,18 @@ __host__ __device__ inline int pow2_ceil(int r) {
     return p;
 }
 
-// Matrix-core repulsion (bf16 tables): a batch is processed in groups of
-// kMfI*16 positions and kMfU*16 unique negatives.  Its LDS holds the batch
-// snapshot (kMfI*16 rows of ld + kMfPad bf16; the pad spreads the four row
-// groups of a transposed read over distinct banks) and one 16 x 16 bf16 tile
-// for transposing the gradient (rows of kMfTS).
+// Matrix-core repulsion (bf16 tables): a batch of at most kMfI*16 positions
+// is processed in groups of kMfU*16 unique negatives.  Its LDS holds the batch
+// snapshot (B rows of ld + kMfPad bf16 and one zero row that tile rows past
+// the batch read; the pad spreads the four row groups of a transposed read
+// over distinct banks) and one 16 x 16 bf16 tile for transposing the gradient
+// (rows of kMfTS).
 constexpr int kMfU = 5;
 constexpr int kMfI = 2;
 constexpr int kMfPad = 4;
 constexpr int kMfTS = 20;
 __host__ __device__ inline int mf_snap_stride(int ld) { return ld + kMfPad; }
-inline int mf_bytes(int ld) { return kMfI * 16 * mf_snap_stride(ld) * 2 + 16 * kMfTS * 2; }
+inline int mf_bytes(int ld, int B) { return (B + 1) * mf_snap_stride(ld) * 2 + 16 * kMfTS * 2; }
 
 // The region a wave's LDS starts with: the attraction ring (fp32 value +
 // loaded value per pair and slot) and, reusing it, the repulsion snapshot.
@@ -109,7 +110,7 @@ inline int fast_lds_region(int R, int nq, int bf16, int B, int ld, int mfma) {
     // beyond the last slot, which must stay inside this wave's region
     int ring = R * nq * (8 + e) + 64 * 8;
     int snap = B * nq * e;
-    if (mfma && mf_bytes(ld) > snap) snap = mf_bytes(ld);
+    if (mfma && mf_bytes(ld, B) > snap) snap = mf_bytes(ld, B);
     const int region = ring > snap ? ring : snap;
     return (region + 15) & ~15;
 }
