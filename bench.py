@@ -242,7 +242,10 @@ def main() -> int:
                      for k, r in results.items()},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(head["kernel"]) if w.name == "1b" else None,
+                     # counter profile of the default workload and chunking only
+                     "traffic": (measured_traffic(head["kernel"])
+                                 if args.workload == "1b" and args.scale == 1.0 and args.chunk_words == 10_000_000
+                                 else None),
                      "algorithmic_bytes_per_launch": round(head["alg_bytes"] / max(head["launches"], 1)),
                      "kernel": f"sgns_fast_kernel ({head['kernel']} repulsion)",
                      "avg_launch_ms": round(head["avg_launch_ms"], 4)},
