@@ -448,10 +448,10 @@ static int ensure_order(par_sgns_handle h, int64_t b, int64_t e) {
 // resident wave of the chip turns a pass into a few huge stale steps.
 constexpr int64_t kMinSentencesPerWave = 16;
 
-// Resident blocks per CU for a block of wpb waves: 160 KB of LDS per CU, each
-// block also holding the sigmoid table; at most kMaxWavesCu waves per CU
+// Resident blocks per CU for a block of wpb waves: 160 KB of LDS per CU (the
+// kernel has no per-block LDS of its own); at most kMaxWavesCu waves per CU
 // (two per SIMD at the kernel's register budget).
-constexpr int kLdsCu = 160 * 1024, kBlockStatic = 4096, kMaxWavesCu = 8, kMaxWpb = 8;
+constexpr int kLdsCu = 160 * 1024, kBlockStatic = 0, kMaxWavesCu = 8, kMaxWpb = 8;
 static int blocks_per_cu(int lds_per_wave, int wpb) {
     int b = kLdsCu / (lds_per_wave * wpb + kBlockStatic);
     if (b * wpb > kMaxWavesCu) b = kMaxWavesCu / wpb;

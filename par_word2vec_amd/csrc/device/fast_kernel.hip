@@ -174,7 +174,7 @@ __device__ __forceinline__ void wb_quad(uint16_t *w, int row, int ld, int jD, in
 template <int NP>
 __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t *bat, int isz, int U,
                                                const int32_t *uneg, const int32_t *umul, char *lds, int lane,
-                                               float alpha_r, double &loss_r, uint32_t &rs, const float *et) {
+                                               float alpha_r, double &loss_r, uint32_t &rs) {
     uint16_t *w_in = static_cast<uint16_t *>(a.w_in);
     uint16_t *w_out = static_cast<uint16_t *>(a.w_out);
     const int ld = a.ld, d = a.d, SC = mf_snap_stride(ld);
@@ -266,7 +266,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                         if (ul < U && il < ni) {
                             const float f = acc[it][r];
                             const float m = (float)umul[ul];
-                            gm = grad_label0(f, alpha_r, et) * m;
+                            gm = grad_label0_r(f, alpha_r) * m;
                             if (a.compute_loss) lr += (double)m * (double)softplusf(f);
                         }
                         gb[r] = bf16_rne(gm);
@@ -363,15 +363,19 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                     bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, off + 2u * (uint32_t)jD, 0, kSc1);
                 }
             };
-            u32x2_t bnext[kMfU];
-            load_base(0, bnext);
+            u32x2_t bn0[kMfU], bn1[kMfU];  // two steps ahead
+            load_base(0, bn0);
+            if (1 < JT) load_base(1, bn1);
             for (int jt = 0; jt < JT; jt++) {
                 const int jA = jt * 16 + r16;
                 const int jD = jt * 16 + 4 * g4;
                 u32x2_t bcur[kMfU];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) bcur[ut] = bnext[ut];
-                if (jt + 1 < JT) load_base(jt + 1, bnext);
+                for (int ut = 0; ut < kMfU; ut++) {
+                    bcur[ut] = bn0[ut];
+                    bn0[ut] = bn1[ut];
+                }
+                if (jt + 2 < JT) load_base(jt + 2, bn1);
                 f32x4_t accN[kMfU];
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) accN[ut] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -412,7 +416,7 @@ struct SentCount {
 // processed so far (word_count_actual, cpu:431-445), not with corpus position.
 template <typename T, int NP, int NB, bool MF>
 __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int64_t raw, int lane, char *lds, double &loss_a,
-                               double &loss_r, SentCount &cnt, const float *et) {
+                               double &loss_r, SentCount &cnt) {
     using PT = typename Pair<T>::P;
     T *w_in = static_cast<T *>(a.w_in);
     T *w_out = static_cast<T *>(a.w_out);
@@ -612,7 +616,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                     part += u[t].x * v[t].x + u[t].y * v[t].y;
                 }
                 const float f = wave_sum(part);
-                const float g = grad_label1(f, alpha_p, et);
+                const float g = grad_label1_r(f, alpha_p);
 #pragma unroll
                 for (int t = 0; t < NP; t++) {
                     float2 nw = make_float2(u[t].x + g * v[t].x, u[t].y + g * v[t].y);
@@ -750,7 +754,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             U += __builtin_popcountll(fm);
         }
         if constexpr (MF) {  // matrix-core path (bf16 tables, its own kernel instance)
-            repulsion_mfma<NP>(a, sen + lo, isz, U, uneg, umul, lds, lane, alpha_r, loss_r, rs, et);
+            repulsion_mfma<NP>(a, sen + lo, isz, U, uneg, umul, lds, lane, alpha_r, loss_r, rs);
             continue;
         }
         // snapshot of the batch rows (after this sentence's attraction), four in flight
@@ -814,7 +818,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
 #pragma unroll
                     for (int t = 0; t < NP; t++) part += nv[k][t].x * c[t].x + nv[k][t].y * c[t].y;
                     const float f = wave_sum(part);
-                    g[k] = k < nbk ? grad_label0(f, alpha_r, et) : 0.0f;
+                    g[k] = k < nbk ? grad_label0_r(f, alpha_r) : 0.0f;
                     fbuf = lane == k ? f : fbuf;
                 }
                 if (a.compute_loss && lane < nbk) loss_r += (double)mlane * (double)softplusf(fbuf);
@@ -854,9 +858,6 @@ template <typename T, int NP, int NB, bool MF>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void sgns_fast_kernel(FastArgs a) {
     static_assert(!MF || sizeof(T) == 2, "the matrix-core repulsion runs on bf16 tables");
     extern __shared__ __align__(16) char smem[];
-    __shared__ float et[kExpTable + 1];  // sigmoid table, one copy per workgroup
-    for (int i = threadIdx.x; i <= kExpTable; i += blockDim.x) et[i] = a.exp_table[i];
-    __syncthreads();
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     char *lds = smem + (size_t)wave * a.lds_per_wave;
     double loss_a = 0.0, loss_r = 0.0;
@@ -882,7 +883,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void s
             const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
             const int64_t raw = a.plan_b ? 0 : (a.order ? a.queue_raw[slot] : a.sent_raw[s]);
             SentCount c{0u, 0u, 0u, 0u, 0ull};
-            train_sentence<T, NP, NB, MF>(a, s, raw, lane, lds, loss_a, loss_r, c, et);
+            train_sentence<T, NP, NB, MF>(a, s, raw, lane, lds, loss_a, loss_r, c);
             fold(c);
         }
     } else {
@@ -890,7 +891,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void s
         const int64_t s = a.s_begin + idx;
         if (idx < n) {
             SentCount c{0u, 0u, 0u, 0u, 0ull};
-            train_sentence<T, NP, NB, MF>(a, s, a.plan_b ? 0 : a.sent_raw[s], lane, lds, loss_a, loss_r, c, et);
+            train_sentence<T, NP, NB, MF>(a, s, a.plan_b ? 0 : a.sent_raw[s], lane, lds, loss_a, loss_r, c);
             fold(c);
         }
     }

@@ -142,6 +142,27 @@ __device__ __forceinline__ float grad_label0(float f, float alpha, const float *
     return (0.0f - et[(int)((f + (float)kMaxExp) * (1000.0f / 12.0f))]) * alpha;
 }
 
+// The same rule with the table entry computed in registers instead of read
+// from LDS: entry i is e/(e+1) = 1/(1 + exp(-x_i)) at x_i = (i/1000*2-1)*6
+// (cpu:822-826), here with the hardware exp and reciprocal, which can differ
+// from the table's bits in the last places.  Used by the Hogwild kernel,
+// where the lookup's LDS round trip sits in every pair's dependent chain.
+__device__ __forceinline__ float sig_entry(float f) {
+    const int i = (int)((f + (float)kMaxExp) * (1000.0f / 12.0f));
+    const float x = (float)i * (2.0f * kMaxExp / kExpTable) - (float)kMaxExp;
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-x));
+}
+__device__ __forceinline__ float grad_label1_r(float f, float alpha) {
+    if (f > kMaxExp) return 0.0f;
+    if (f < -kMaxExp) return alpha;
+    return (1.0f - sig_entry(f)) * alpha;
+}
+__device__ __forceinline__ float grad_label0_r(float f, float alpha) {
+    if (f > kMaxExp) return -alpha;
+    if (f < -kMaxExp) return 0.0f;
+    return (0.0f - sig_entry(f)) * alpha;
+}
+
 __device__ __forceinline__ float softplusf(float x) {  // log(1 + e^x)
     return x > 0.0f ? x + log1pf(__expf(-x)) : log1pf(__expf(x));
 }
