@@ -546,7 +546,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
             return seterr(h, PAR_SGNS_EINVAL, "matrix-core repulsion runs on bf16 tables below 4 GiB with batch size <= 32");
         f.rep_mfma = h->bf16 && small && a->repulsion != PAR_SGNS_REP_VECTOR;
         f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld, f.rep_mfma);
-        f.lds_region = sgns::fast_lds_region(h->R, h->nq, h->bf16, h->cfg.batch_size, h->ld, f.rep_mfma);
+        f.lds_negs = sgns::fast_lds_negs(h->nq, h->bf16, h->cfg.batch_size, h->ld, f.rep_mfma);
         f.persistent = pers ? 1 : 0; f.rounding = h->cfg.rounding; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
         f.a0 = a->alpha; f.T = a->train_words; f.seed = a->seed; f.s_begin = a->sent_begin; f.s_end = a->sent_end;
         f.queue = h->queue; f.loss = h->loss; f.counters = h->counters;

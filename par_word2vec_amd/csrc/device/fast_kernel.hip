@@ -51,9 +51,8 @@ __device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, fl
     if (dlt.x != 0.0f) unsafeAtomicAdd(p, dlt.x);
     if (dlt.y != 0.0f) unsafeAtomicAdd(p + 1, dlt.y);
 }
-__device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 base, float2 dlt,
-                                        uint32_t &rs, int nearest) {
-    if (dlt.x == 0.0f && dlt.y == 0.0f) return;
+// The packed bf16 step s - base (0 if the pair does not move).
+__device__ __forceinline__ uint32_t bf16_step(float2 base, float2 dlt, uint32_t &rs, int nearest) {
     rs ^= rs << 13;  // per-lane xorshift32 stream (Marsaglia 2003) for rounding noise
     rs ^= rs >> 17;
     rs ^= rs << 5;
@@ -67,8 +66,13 @@ __device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q,
     const uint32_t rx = (bx + ax) >> 16, ry = (by + ay) >> 16;
     const float sx = __builtin_bit_cast(float, rx << 16) - base.x;
     const float sy = __builtin_bit_cast(float, ry << 16) - base.y;
-    if (sx == 0.0f && sy == 0.0f) return;
-    const uint32_t step = bf16_rne(sx) | (bf16_rne(sy) << 16);
+    return bf16_rne(sx) | (bf16_rne(sy) << 16);  // exact: a multiple of base's ulp
+}
+__device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 base, float2 dlt,
+                                        uint32_t &rs, int nearest) {
+    if (dlt.x == 0.0f && dlt.y == 0.0f) return;
+    const uint32_t step = bf16_step(base, dlt, rs, nearest);
+    if (step == 0u) return;
     bf16x2_t *p = reinterpret_cast<bf16x2_t *>(w + row * ld) + q;
     __builtin_amdgcn_global_atomic_fadd_v2bf16(p, __builtin_bit_cast(bf16x2_t, step));
 }
@@ -130,25 +134,51 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 
-// W_out reads of the matrix-core path go through a buffer resource: 32-bit
+// Table reads of the matrix-core path go through buffer resources: 32-bit
 // byte offsets (the host enables the path only for tables below 4 GiB - 4 KiB)
 // and the agent-coherent cache policy (sc1) of the row loads.  A lane with no
-// row reads at kNoRow, past the resource's size: the hardware range check
-// returns zero, so every load is unconditional (a lane-predicated load whose
-// result is merged into a packed register gets a wait of its own).
+// row reads at kNoRow (+ less than 4 KiB), past the resource's size: the
+// hardware range check returns zero.  So every load is unconditional: a load
+// under a branch, or whose result is merged with a default, makes the
+// compiler wait for it (and, since a wave's memory operations retire in
+// order, for everything issued before it) where the values merge, and loads
+// issued ahead of their use would gain nothing.  Operand buffers for steps
+// ahead alternate between two register sets (no rotating copies, which would
+// wait for the load just issued), and a scheduling barrier keeps each step's
+// loads ahead of its write-back atomics (buffer loads and global atomics are
+// otherwise free to be reordered).
 constexpr int kSc1 = 16;
 constexpr int kRsrcWord3 = 0x00020000;
 constexpr uint32_t kNoRow = 0xFFFFF000u;
+
+// LLVM's packed-bf16 buffer atomic add (buffer_atomic_pk_add_bf16), bound
+// by name: clang has no builtin for it.  Its result is never used (the
+// no-return form is emitted).
+__device__ bf16x2_t buffer_atomic_pk_add_bf16(bf16x2_t v, __amdgpu_buffer_rsrc_t r, int off, int soff, int aux)
+    __asm("llvm.amdgcn.raw.ptr.buffer.atomic.fadd.v2bf16");
 
 // Write back one lane's output dimensions jD..jD+3 of row `row` (D layout of a
 // 16x16 tile: lane r of a 16-lane row holds tile row r).  Lanes r and r^8
 // swap halves (DPP row_ror:8) so that the first atomic covers tile rows 0..7
 // with 32 contiguous bytes each and the second rows 8..15: one memory request
-// per row and instruction instead of two.  Called with every lane active; a
-// lane with row < 0 or jD >= d (d even) writes nothing.
+// per row and instruction instead of two.  quad_steps() computes the two
+// packed steps and their byte offsets (called with every lane active, or
+// skipped for an empty tile); issue_quad() sends both atomics from every lane,
+// every time: a pair with nothing to add (no row, past d, a step that rounds
+// to zero, an empty tile) goes to kNoRow, which the range check drops without
+// a memory request.  Unconditional atomics keep the compiler's wait counts
+// exact, so a wait for a load never also waits for atomics issued after it.
+struct Quad {
+    uint32_t sa, sr;  // packed bf16 steps: tile rows 0..7, rows 8..15
+    uint32_t oa, orr; // their byte offsets (kNoRow: nothing to add)
+};
+__device__ __forceinline__ void issue_quad(__amdgpu_buffer_rsrc_t rw, const Quad &qd) {
+    buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, qd.sa), rw, (int)qd.oa, 0, 0);
+    buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, qd.sr), rw, (int)qd.orr, 0, 0);
+}
 __device__ __forceinline__ int dpp_ror8_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false); }
-__device__ __forceinline__ void wb_quad(uint16_t *w, int row, int ld, int jD, int d, uint32_t b01, uint32_t b23,
-                                        f32x4_t acc, bool lo, uint32_t &rs, int nearest) {
+__device__ __forceinline__ Quad quad_steps(int row, uint32_t row_bytes, int jD, int d, uint32_t b01, uint32_t b23,
+                                           f32x4_t acc, bool lo, uint32_t &rs, int nearest) {
     // lo lanes send their second pair, hi lanes their first
     const float sx = lo ? acc[2] : acc[0], sy = lo ? acc[3] : acc[1];
     const uint32_t sb = lo ? b23 : b01;
@@ -156,19 +186,20 @@ __device__ __forceinline__ void wb_quad(uint16_t *w, int row, int ld, int jD, in
     const float ry = __builtin_bit_cast(float, dpp_ror8_i(__builtin_bit_cast(int, sy)));
     const uint32_t rb = (uint32_t)dpp_ror8_i((int)sb);
     const int prow = dpp_ror8_i(row);
-    if (jD >= d) return;
     const int q = (jD >> 1) + (lo ? 0 : 1);  // lo lanes write the first pair, hi lanes the second
-    const bool inside = lo || jD + 2 < d;
+    const bool inside = 2 * q < d;           // the pair starts below d
     // rows 0..7 of the tile
     const int ra = lo ? row : prow;
-    if (ra >= 0 && inside)
-        wb_pair(w, ra, ld, q, Pair<uint16_t>::tof(lo ? b01 : rb), lo ? make_float2(acc[0], acc[1]) : make_float2(rx, ry),
-                rs, nearest);
+    Quad qd;
+    qd.sa = bf16_step(Pair<uint16_t>::tof(lo ? b01 : rb), lo ? make_float2(acc[0], acc[1]) : make_float2(rx, ry), rs,
+                      nearest);
+    qd.oa = ra >= 0 && inside && qd.sa != 0u ? (uint32_t)ra * row_bytes + 4u * q : kNoRow;
     // rows 8..15
-    const int rb_row = lo ? prow : row;
-    if (rb_row >= 0 && inside)
-        wb_pair(w, rb_row, ld, q, Pair<uint16_t>::tof(lo ? rb : b23), lo ? make_float2(rx, ry) : make_float2(acc[2], acc[3]),
-                rs, nearest);
+    const int rr = lo ? prow : row;
+    qd.sr = bf16_step(Pair<uint16_t>::tof(lo ? rb : b23), lo ? make_float2(rx, ry) : make_float2(acc[2], acc[3]), rs,
+                      nearest);
+    qd.orr = rr >= 0 && inside && qd.sr != 0u ? (uint32_t)rr * row_bytes + 4u * q : kNoRow;
+    return qd;
 }
 
 template <int NP>
@@ -181,6 +212,8 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
     const uint32_t row_bytes = (uint32_t)ld * 2u;
     const __amdgpu_buffer_rsrc_t rout =
         __builtin_amdgcn_make_buffer_rsrc(w_out, (short)0, (int)((uint32_t)a.V * row_bytes), kRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(w_in, (short)0, (int)((uint32_t)a.V * row_bytes), kRsrcWord3);
     uint16_t *snapC = reinterpret_cast<uint16_t *>(lds);
     uint16_t *tr = snapC + (a.B + 1) * SC;
     const int r16 = lane & 15, g4 = lane >> 4;
@@ -196,11 +229,10 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
             uint32_t rows[8][NP];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                if (i + k < ni) load_row<uint16_t, NP>(w_in, bat[i + k], ld, a.nq, lane, rows[k]);
-                else {
+                const uint32_t off = (i + k < ni ? (uint32_t)bat[i + k] * row_bytes : kNoRow) + 4u * (uint32_t)lane;
 #pragma unroll
-                    for (int t = 0; t < NP; t++) rows[k][t] = 0u;
-                }
+                for (int t = 0; t < NP; t++)  // lanes past the row read the next row's start (not stored)
+                    rows[k][t] = __builtin_amdgcn_raw_buffer_load_b32(rin, off + 256u * t, 0, kSc1);
             }
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -233,10 +265,8 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 // this tile's 16 rows of N, all of ld: 16 bytes per lane per 32 dimensions
                 u32x4_t nv[4 * NP];
 #pragma unroll
-                for (int ks = 0; ks < 4 * NP; ks++) {
-                    nv[ks] = u32x4_t{0u, 0u, 0u, 0u};
-                    if (ks < KS) nv[ks] = __builtin_amdgcn_raw_buffer_load_b128(rout, offN + 64u * ks, 0, kSc1);
-                }
+                for (int ks = 0; ks < 4 * NP; ks++)
+                    nv[ks] = __builtin_amdgcn_raw_buffer_load_b128(rout, ks < KS ? offN + 64u * ks : kNoRow, 0, kSc1);
                 f32x4_t acc[kMfI];
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) acc[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -293,31 +323,39 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
             }
             // raw 16-bit values, one register each: packing them at load time
             // would make the wave wait for the loads there
+            // (steps past the last read dimensions past d: the next row's or
+            // zeros, multiplied but never written back)
             auto load_a = [&](int jt, uint32_t (&av)[kMfU][4]) {
                 const uint32_t jb = 2u * (uint32_t)(jt * 16 + r16);
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) {
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        av[ut][r] = 0u;
-                        if (ut < UT) av[ut][r] = __builtin_amdgcn_raw_buffer_load_b16(rout, offA[ut][r] + jb, 0, kSc1);
-                    }
+                    for (int r = 0; r < 4; r++)
+                        av[ut][r] = __builtin_amdgcn_raw_buffer_load_b16(rout, offA[ut][r] + jb, 0, kSc1);
                 }
             };
-            // two steps in flight ahead of the one being computed
-            uint32_t an0[kMfU][4], an1[kMfU][4];
-            load_a(0, an0);
-            if (1 < JT) load_a(1, an1);
-            for (int jt = 0; jt < JT; jt++) {
+            // steps in pairs (JT rounded up to even), operands two steps ahead
+            uint32_t aX[kMfU][4], aY[kMfU][4];
+            // (each followed by the atomics a step issues, all dropped: the loop is
+            // then entered with the operations in flight of every later step,
+            // and its waits are counted for that, not for a shorter prologue)
+            const Quad none{0u, 0u, kNoRow, kNoRow};
+            load_a(0, aX);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int it = 0; it < kMfI; it++) issue_quad(rin, none);
+            load_a(1, aY);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int it = 0; it < kMfI; it++) issue_quad(rin, none);
+            auto step_c = [&](int jt, uint32_t (&av)[kMfU][4]) {
                 const int jD = jt * 16 + 4 * g4;  // first of this lane's four output dimensions
                 s16x4_t ac[kMfU];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-                    ac[ut] = s16x4_t{(short)an0[ut][0], (short)an0[ut][1], (short)an0[ut][2], (short)an0[ut][3]};
-#pragma unroll
-                    for (int r = 0; r < 4; r++) an0[ut][r] = an1[ut][r];
-                }
-                if (jt + 2 < JT) load_a(jt + 2, an1);
+                for (int ut = 0; ut < kMfU; ut++)
+                    ac[ut] = s16x4_t{(short)av[ut][0], (short)av[ut][1], (short)av[ut][2], (short)av[ut][3]};
+                load_a(jt + 2, av);
+                __builtin_amdgcn_sched_barrier(0);
                 f32x4_t accC[kMfI];
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) accC[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -332,10 +370,17 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 }
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) {
-                    if (it >= IT) continue;  // uniform
-                    const uint2 b = *reinterpret_cast<const uint2 *>(snapC + srow(it * 16 + r16) * SC + jD);
-                    wb_quad(w_in, rowC[it], ld, jD, d, b.x, b.y, accC[it], r16 < 8, rs, a.rounding);
+                    Quad qd{0u, 0u, kNoRow, kNoRow};
+                    if (it < IT) {  // uniform
+                        const uint2 b = *reinterpret_cast<const uint2 *>(snapC + srow(it * 16 + r16) * SC + jD);
+                        qd = quad_steps(rowC[it], row_bytes, jD, d, b.x, b.y, accC[it], r16 < 8, rs, a.rounding);
+                    }
+                    issue_quad(rin, qd);
                 }
+            };
+            for (int jt = 0; jt < JT; jt += 2) {
+                step_c(jt, aX);
+                step_c(jt + 1, aY);
             }
             // ---- Gm^T: each 16 x 16 tile transposed through LDS
             s16x4_t gt[kMfU][kMfI];
@@ -353,29 +398,36 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
             }
             // ---- dN^T = C^T Gm^T, sixteen dimensions at a time; the current
             // values the write-backs round against are fetched one step ahead
-            auto load_base = [&](int jt, u32x2_t (&bv)[kMfU]) {
-                const int jD = jt * 16 + 4 * g4;
+            // (rowN = -1 past U; dimensions past d read the row's zero padding or
+            // the next row, never written back)
+            uint32_t offB[kMfU];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-                    bv[ut] = u32x2_t{0u, 0u};
-                    if (ut >= UT) continue;  // uniform; dimensions past d read the row's zero padding
-                    const uint32_t off = rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes : kNoRow;
-                    bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, off + 2u * (uint32_t)jD, 0, kSc1);
-                }
+            for (int ut = 0; ut < kMfU; ut++) offB[ut] = rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes : kNoRow;
+            auto load_base = [&](int jt, u32x2_t (&bv)[kMfU]) {
+                const uint32_t jb = 2u * (uint32_t)(jt * 16 + 4 * g4);
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++)
+                    bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, offB[ut] + jb, 0, kSc1);
             };
-            u32x2_t bn0[kMfU], bn1[kMfU];  // two steps ahead
-            load_base(0, bn0);
-            if (1 < JT) load_base(1, bn1);
-            for (int jt = 0; jt < JT; jt++) {
+            u32x2_t bX[kMfU], bY[kMfU];  // steps in pairs, two steps ahead
+            load_base(0, bX);  // (as for dC: each followed by a step's dropped atomics)
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int ut = 0; ut < kMfU; ut++) issue_quad(rout, none);
+            load_base(1, bY);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int ut = 0; ut < kMfU; ut++) issue_quad(rout, none);
+            auto step_n = [&](int jt, u32x2_t (&bv)[kMfU]) {
                 const int jA = jt * 16 + r16;
                 const int jD = jt * 16 + 4 * g4;
+                // this step's values out of bv (explicit copies, so the loads
+                // below can reuse its registers), then the step two ahead
                 u32x2_t bcur[kMfU];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-                    bcur[ut] = bn0[ut];
-                    bn0[ut] = bn1[ut];
-                }
-                if (jt + 2 < JT) load_base(jt + 2, bn1);
+                for (int ut = 0; ut < kMfU; ut++) bcur[ut] = u32x2_t{copy_u32(bv[ut].x), copy_u32(bv[ut].y)};
+                load_base(jt + 2, bv);
+                __builtin_amdgcn_sched_barrier(0);
                 f32x4_t accN[kMfU];
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) accN[ut] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
@@ -393,9 +445,16 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 }
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) {
-                    if (ut >= UT) continue;  // uniform
-                    wb_quad(w_out, rowN[ut], ld, jD, d, bcur[ut].x, bcur[ut].y, accN[ut], r16 < 8, rs, a.rounding);
+                    Quad qd{0u, 0u, kNoRow, kNoRow};
+                    if (ut < UT)  // uniform
+                        qd = quad_steps(rowN[ut], row_bytes, jD, d, bcur[ut].x, bcur[ut].y, accN[ut], r16 < 8, rs,
+                                        a.rounding);
+                    issue_quad(rout, qd);
                 }
+            };
+            for (int jt = 0; jt < JT; jt += 2) {
+                step_n(jt, bX);
+                step_n(jt + 1, bY);
             }
         }
     }
@@ -437,21 +496,26 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     // shared negatives
     float2 *ring_u = reinterpret_cast<float2 *>(lds);
     PT *ring_0 = reinterpret_cast<PT *>(lds + (size_t)R * nq * sizeof(float2));
-    int32_t *negs = reinterpret_cast<int32_t *>(lds + a.lds_region);
+    int32_t *negs = reinterpret_cast<int32_t *>(lds + a.lds_negs);
     // Slot tables of the ring, one entry per lane (R = 2W+1 <= 64): lane k
     // holds the word of slot k, the last window position referencing slot k,
     // and the slot of the positions c with (c & pmask) == k.
     const int pmask = pow2_ceil(R) - 1;
     int slot_word = -1, slot_last = kUnusedSlot, pos_slot = 0;
-    // all ones if this lane's last pair lies inside the row
-    // (kept opaque so it stays a per-lane bit mask, not a scalar lane predicate)
-    uint32_t last_mask = lane + kWave * (NP - 1) < nq ? 0xFFFFFFFFu : 0u;
-    asm volatile("" : "+v"(last_mask));
     // byte stride of a ring slot, in a vector register: the pair loop's
     // address is then one multiply-add with no scalar reload of nq
     uint32_t slot_bytes = (uint32_t)nq * (uint32_t)sizeof(float2);
     asm volatile("" : "+v"(slot_bytes));
     char *ring_lane = reinterpret_cast<char *>(ring_u + lane);
+    // A lane whose last pair lies past the row (q >= nq) points that pair at a
+    // zero pair of its own in the slack after the ring, for every slot: its
+    // centre-row value is zero too, so it reads 0 and writes 0 there, and the
+    // pair loop needs no lane predicate or mask.
+    const bool last_in = lane + kWave * (NP - 1) < nq;
+    float2 *zero_pair = reinterpret_cast<float2 *>(lds + (size_t)R * nq * (sizeof(float2) + sizeof(PT))) + lane;
+    uint32_t slot_bytes_last = last_in ? slot_bytes : 0u;
+    asm volatile("" : "+v"(slot_bytes_last));
+    char *ring_last = last_in ? ring_lane + kWave * sizeof(float2) * (NP - 1) : reinterpret_cast<char *>(zero_pair);
 
     cnt.tokens = (uint32_t)L;
 
@@ -480,17 +544,29 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             }
         }
     };
-    auto flush = [&](int slot) {
-        const int64_t row = word_of(slot);
+    // a slot's current and loaded values, then their write-back to W_in
+    auto read_slot = [&](int slot, float2 (&u)[NP], float2 (&u0)[NP]) {
 #pragma unroll
         for (int t = 0; t < NP; t++) {
             const int q = lane + kWave * t;
             if (t < NP - 1 || q < nq) {
-                const float2 u = ring_u[slot * nq + q];
-                const float2 u0 = Pair<T>::tof(ring_0[slot * nq + q]);
-                wb_pair(w_in, row, ld, q, u0, make_float2(u.x - u0.x, u.y - u0.y), rs, a.rounding);
+                u[t] = ring_u[slot * nq + q];
+                u0[t] = Pair<T>::tof(ring_0[slot * nq + q]);
             }
         }
+    };
+    auto write_slot = [&](int64_t row, const float2 (&u)[NP], const float2 (&u0)[NP]) {
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            const int q = lane + kWave * t;
+            if (t < NP - 1 || q < nq)
+                wb_pair(w_in, row, ld, q, u0[t], make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y), rs, a.rounding);
+        }
+    };
+    auto flush = [&](int slot) {
+        float2 u[NP], u0[NP];
+        read_slot(slot, u, u0);
+        write_slot(word_of(slot), u, u0);
     };
     auto bind = [&](int c, int slot, int w) {
         if (lane == slot) {
@@ -505,13 +581,16 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     // word occurring twice in the window shares one slot, so every pair sees
     // the updates of the pairs before it exactly as cpu:551-565 does in place.
     // A slot is written back when the last position referencing it leaves the
-    // window.  The W_out row of centre p+2 and the W_in row entering at p+2+W
-    // are loaded while centre p trains, except when this wave may still write
-    // that row first (the W_out row of the current centre's word; a W_in row
-    // whose word holds a slot that could be flushed): those are loaded after
-    // centre p's write-backs, so every pair sees the same values as in the
-    // reference's sequential order.
+    // window.  At the end of centre p the slot bookkeeping and the fill of the
+    // entering row come first (their operands were loaded a centre earlier),
+    // then this centre's write-backs, then the loads for two centres ahead:
+    // the W_out row of centre p+2 and the W_in row entering at p+2+W.  Loaded
+    // after every write-back this wave makes before using them, they hold the
+    // values of the reference's sequential order; and since a wave's memory
+    // operations retire in issue order, every wait in the loop is for
+    // operations issued a full centre earlier.
     {
+        *zero_pair = make_float2(0.0f, 0.0f);  // (the repulsion may have used these bytes)
         // the sentence's tokens [tb, tb+128) in two registers, lane k holding
         // tokens tb+k and tb+64+k; every token the centre loop needs lies there
         int tb = 0;
@@ -548,12 +627,17 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                     if (sl4[k] >= 0) fill(sl4[k], rows[k]);
             }
         }
-        // vq0/vq1: W_out rows of centres p and p+1; pq0/pq1: W_in rows entering
-        // at p+1+W and p+2+W (used only if the word then holds no slot)
-        PT vq0[NP], vq1[NP], pq0[NP], pq1[NP];
+        // vq0/vq1: W_out rows of centres p and p+1; pq: the W_in row entering at
+        // p+1+W, loaded only if its word then holds no slot (np_slot < 0)
+        PT vq0[NP], vq1[NP], pq[NP];
         load_row<T, NP>(w_out, tok(0), ld, nq, lane, vq0);
-        if (1 < L) load_row<T, NP>(w_out, tok(1), ld, nq, lane, vq1);
-        if (1 + W < L) load_row<T, NP>(w_in, tok(1 + W), ld, nq, lane, pq0);
+        if (1 < L && tok(1) != tok(0)) load_row<T, NP>(w_out, tok(1), ld, nq, lane, vq1);
+        int np_slot = 1 + W < L ? find_slot(tok(1 + W), -W) : -1;
+        if (1 + W < L && np_slot < 0) load_row<T, NP>(w_in, tok(1 + W), ld, nq, lane, pq);
+        // centre 0's row, settled here: every iteration then enters the loop
+        // with the same loads in flight (the next centres' rows only)
+#pragma unroll
+        for (int t = 0; t < NP; t++) vq0[t] = Pair<T>::copy(vq0[t]);
         float2 v[NP], v0[NP];
         bool carried = false;
         uint32_t bwin = 0;
@@ -564,8 +648,9 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 if (p > 0) {  // slide the token window: [p, p+128)
                     tb = p;
                     tw0 = tw1;
-                    tw1 = tok_block(p + kWave);
+                    tw1 = (int)copy_u32((uint32_t)tok_block(p + kWave));
                 }
+                bwin = copy_u32(bwin);
             }
             const int b = __builtin_amdgcn_readlane((int)bwin, p & (kWave - 1));
             const float alpha_p = p == 0 ? alpha0 : alpha;
@@ -579,86 +664,139 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             const bool carry_next = wn == wp;
             const int np = p + 1 + W;
             const int wnp = np < L ? tok(np) : -1;
-            const int np_slot = np < L ? find_slot(wnp, p - W) : -1;
-            // loads for two centres ahead that this centre's write-backs cannot affect
-#pragma unroll
-            for (int t = 0; t < NP; t++) vq0[t] = vq1[t];
-            const int w2 = p + 2 < L ? tok(p + 2) : -1;
-            const bool v_late = w2 >= 0 && w2 == wp;  // written back at the end of this centre
-            if (w2 >= 0 && !v_late) load_row<T, NP>(w_out, w2, ld, nq, lane, vq1);
-            const int e2 = np + 1 < L ? tok(np + 1) : -1;
-            const bool p_late = e2 >= 0 && find_slot(e2, p - W) >= 0;  // its slot may be flushed first
-            if (e2 >= 0 && !p_late) load_row<T, NP>(w_in, e2, ld, nq, lane, pq1);
+            // Pairs of centre p: positions lo..hi except p (cpu:547-550), in
+            // order.  Pair k's logit f_k = u_k . v_k, with u_k its ring row and
+            // v_k the centre row after k pairs, is assembled as
+            //   f_k = u_k . v_{k-1} + g_{k-1} (u_k . u_{k-1})
+            // (v_k = v_{k-1} + g_{k-1} u_{k-1}), so both products and their
+            // reductions run while g_{k-1} is still being computed; only a
+            // pair reading the slot the previous pair just wrote (the same
+            // word at adjacent positions) takes its logit after that write.
+            const int lo = p - W + b > 0 ? p - W + b : 0;
+            const int hi = p + W - b < L - 1 ? p + W - b : L - 1;
+            const int npair = hi - lo;
             float fbuf = 0.0f;  // lane k: logit of the k-th pair (loss only)
-            int npair = 0;
-            for (int i = b; i < 2 * W + 1 - b; i++) {
-                if (i == W) continue;
-                const int c = p - W + i;
-                if (c < 0) continue;
-                if (c >= L) break;
-                const int slot = slot_of(c);
-                float2 *ru = reinterpret_cast<float2 *>(ring_lane + (uint32_t)slot * slot_bytes);
-                // Ring accesses without lane predicates: a lane past the row
-                // (its last pair only) reads the next entries of the ring, uses
-                // them as zero (integer mask, whatever the bits) and writes the
-                // bits it read back unchanged.  The ring region has slack for
-                // the last slot.
-                float2 raw[NP], u[NP];
-                float part = 0.0f;
+            if (npair > 0) {
+                auto pos = [&](int k) { return lo + k + (lo + k >= p ? 1 : 0); };
+                // ring entry t of this lane in a slot (the last one: see zero_pair)
+                auto ring_at = [&](int slot, int t) {
+                    return t == NP - 1 ? reinterpret_cast<float2 *>(ring_last + (uint32_t)slot * slot_bytes_last)
+                                       : reinterpret_cast<float2 *>(ring_lane + (uint32_t)slot * slot_bytes) + kWave * t;
+                };
+                auto read_row = [&](int slot, float2 (&u)[NP]) {
 #pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    raw[t] = ru[kWave * t];
-                    u[t] = raw[t];
-                    if (t == NP - 1) {
-                        u[t].x = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, raw[t].x) & last_mask);
-                        u[t].y = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, raw[t].y) & last_mask);
-                    }
-                    part += u[t].x * v[t].x + u[t].y * v[t].y;
-                }
-                const float f = wave_sum(part);
-                const float g = grad_label1_r(f, alpha_p);
+                    for (int t = 0; t < NP; t++) u[t] = *ring_at(slot, t);
+                };
+                auto dot = [&](const float2 (&x)[NP], const float2 (&y)[NP]) {
+                    float s = 0.0f;
 #pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    float2 nw = make_float2(u[t].x + g * v[t].x, u[t].y + g * v[t].y);
-                    if (t == NP - 1) {
-                        const uint32_t m = last_mask;
-                        nw.x = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, nw.x) & m) |
-                                                             (__builtin_bit_cast(uint32_t, raw[t].x) & ~m));
-                        nw.y = __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, nw.y) & m) |
-                                                             (__builtin_bit_cast(uint32_t, raw[t].y) & ~m));
+                    for (int t = 0; t < NP; t++) s += x[t].x * y[t].x + x[t].y * y[t].y;
+                    return s;
+                };
+                int slot = slot_of(pos(0));
+                float2 u[NP], up[NP];
+                read_row(slot, u);
+#pragma unroll
+                for (int t = 0; t < NP; t++) up[t] = make_float2(0.0f, 0.0f);
+                float A = wave_sum(dot(u, v)), B = 0.0f, gp = 0.0f;
+                // one pair; `more`: pair k+1 exists and its row is read here
+                auto step = [&](int k, auto more) {
+#pragma unroll
+                    for (int t = 0; t < NP; t++) {  // v_k
+                        v[t].x += gp * up[t].x;
+                        v[t].y += gp * up[t].y;
                     }
-                    ru[kWave * t] = nw;
-                    v[t].x += g * u[t].x;
-                    v[t].y += g * u[t].y;
+                    const float f = A + gp * B;
+                    int slot_n = slot;
+                    float2 un[NP];
+                    float An = 0.0f, Bn = 0.0f;
+                    if constexpr (decltype(more)::value) {
+                        slot_n = slot_of(pos(k + 1));
+                        read_row(slot_n, un);
+                        An = wave_sum(dot(un, v));
+                        Bn = wave_sum(dot(un, u));
+                    }
+                    const float g = grad_label1_r(f, alpha_p);
+                    float2 nw[NP];
+#pragma unroll
+                    for (int t = 0; t < NP; t++) {
+                        nw[t] = make_float2(u[t].x + g * v[t].x, u[t].y + g * v[t].y);
+                        *ring_at(slot, t) = nw[t];
+                    }
+                    fbuf = lane == k ? f : fbuf;  // logit of pair k, kept in lane k
+                    if constexpr (decltype(more)::value) {
+                        if (slot_n == slot) {  // pair k+1 reads the row pair k just wrote
+                            float2 vn[NP];
+#pragma unroll
+                            for (int t = 0; t < NP; t++) {
+                                un[t] = nw[t];
+                                vn[t] = make_float2(v[t].x + g * u[t].x, v[t].y + g * u[t].y);
+                            }
+                            An = wave_sum(dot(un, vn));
+                            Bn = 0.0f;
+                        }
+#pragma unroll
+                        for (int t = 0; t < NP; t++) {
+                            up[t] = u[t];
+                            u[t] = un[t];
+                        }
+                        A = An;
+                        B = Bn;
+                        slot = slot_n;
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NP; t++) {
+                            up[t] = u[t];
+                        }
+                    }
+                    gp = g;
+                };
+                for (int k = 0; k + 1 < npair; k++) step(k, std::true_type{});
+                step(npair - 1, std::false_type{});
+#pragma unroll
+                for (int t = 0; t < NP; t++) {  // the last pair's W_out update
+                    v[t].x += gp * up[t].x;
+                    v[t].y += gp * up[t].y;
                 }
-                // logit of this centre's npair-th pair, kept in lane npair
-                fbuf = lane == npair ? f : fbuf;
-                npair++;
             }
             cnt.pairs += (uint32_t)npair;
             if (a.compute_loss && lane < npair) loss_a += (double)softplusf(-fbuf);  // this lane's share
+            // position p-W leaves the window; position np enters it (a word
+            // still in the window keeps its slot, so bind before the flush test)
+            const int r = p - W;
+            const int r_slot = r >= 0 ? slot_of(r) : -1;
+            if (np < L && np_slot >= 0) bind(np, np_slot, wnp);
+            const bool leave = r_slot >= 0 && last_of(r_slot) == r;
+            float2 fu[NP], fu0[NP];
+            int64_t frow = 0;
+            if (leave) {  // read before the fill, which may reuse the slot
+                frow = word_of(r_slot);
+                read_slot(r_slot, fu, fu0);
+            }
+            if (np < L && np_slot < 0) {
+                const int sl = find_free(p + 1 - W);
+                fill(sl, pq);
+                bind(np, sl, wnp);
+            }
+#pragma unroll
+            for (int t = 0; t < NP; t++) vq0[t] = Pair<T>::copy(vq1[t]);
+            // this centre's write-backs
             if (!carry_next) {
                 float2 dv[NP];
 #pragma unroll
                 for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
                 store_delta<T, NP>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
             }
+            if (leave) write_slot(frow, fu, fu0);
             carried = carry_next;
-            // position p-W leaves the window; position np enters it
-            const int r = p - W;
-            const int r_slot = r >= 0 ? slot_of(r) : -1;
-            if (np < L && np_slot >= 0) bind(np, np_slot, wnp);
-            if (r_slot >= 0 && last_of(r_slot) == r) flush(r_slot);
-            if (np < L && np_slot < 0) {
-                const int sl = find_free(p + 1 - W);
-                fill(sl, pq0);
-                bind(np, sl, wnp);
-            }
-#pragma unroll
-            for (int t = 0; t < NP; t++) pq0[t] = pq1[t];
-            // the loads held back until this centre's write-backs were issued
-            if (v_late) load_row<T, NP>(w_out, w2, ld, nq, lane, vq1);
-            if (p_late) load_row<T, NP>(w_in, e2, ld, nq, lane, pq0);
+            // loads for two centres ahead: the W_out row of centre p+2 (unless
+            // centre p+1 carries its row into it) and the W_in row entering at
+            // p+2+W, unless its word holds a slot at the start of centre p+1
+            const int w2 = p + 2 < L ? tok(p + 2) : -1;
+            if (w2 >= 0 && w2 != wn) load_row<T, NP>(w_out, w2, ld, nq, lane, vq1);
+            const int e2 = np + 1 < L ? tok(np + 1) : -1;
+            np_slot = e2 >= 0 ? find_slot(e2, p + 1 - W) : -1;
+            if (e2 >= 0 && np_slot < 0) load_row<T, NP>(w_in, e2, ld, nq, lane, pq);
         }
         {  // slots still referenced by the last window
             uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last >= L - W);

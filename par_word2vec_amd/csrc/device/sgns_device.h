@@ -108,6 +108,17 @@ __device__ __forceinline__ uint32_t bf16_rne(float x) {
 // ------------------------------------------------------------- row pairs
 // Storage traits: a "pair" is two adjacent elements of a row.
 template <typename T> struct Pair;
+// A register copy made where it is written.  A loop that rotates prefetch
+// buffers (next <- after-next, then a new load into after-next) otherwise
+// gets the copy at its latch, after the new load was issued, and the copy
+// waits for that load.  Also used to take a value loaded inside a rarely
+// taken branch out of the loop's pending loads, so the loop header does not
+// wait for every load in flight.
+__device__ __forceinline__ uint32_t copy_u32(uint32_t p) {
+    uint32_t r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(p));
+    return r;
+}
 // Table rows are read with agent-coherent loads (sc1 on gfx950): every
 // update reaches memory through the L2 (float atomics or coherent stores), so
 // an ordinary load could keep returning a row's stale copy from the CU's
@@ -119,6 +130,11 @@ template <> struct Pair<float> {
         return __builtin_bit_cast(float2, __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
     __device__ static __forceinline__ float2 tof(float2 p) { return p; }
+    __device__ static __forceinline__ float2 copy(float2 p) {
+        float2 r;
+        asm volatile("v_mov_b32 %0, %2\n\tv_mov_b32 %1, %3" : "=&v"(r.x), "=&v"(r.y) : "v"(p.x), "v"(p.y));
+        return r;
+    }
 };
 template <> struct Pair<uint16_t> {  // bf16 bit patterns
     using P = uint32_t;
@@ -127,6 +143,7 @@ template <> struct Pair<uint16_t> {  // bf16 bit patterns
         return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ static __forceinline__ float2 tof(uint32_t p) { return make_float2(bf16lo(p), bf16hi(p)); }
+    __device__ static __forceinline__ uint32_t copy(uint32_t p) { return copy_u32(p); }
 };
 
 // sigmoid-table gradient of a pair, cpu:554-559 (label 1) and cpu:642-647
@@ -148,19 +165,20 @@ __device__ __forceinline__ float grad_label0(float f, float alpha, const float *
 // from the table's bits in the last places.  Used by the Hogwild kernel,
 // where the lookup's LDS round trip sits in every pair's dependent chain.
 __device__ __forceinline__ float sig_entry(float f) {
+    f = fminf(fmaxf(f, -(float)kMaxExp), (float)kMaxExp);  // callers clip outside [-6, 6]
     const int i = (int)((f + (float)kMaxExp) * (1000.0f / 12.0f));
     const float x = (float)i * (2.0f * kMaxExp / kExpTable) - (float)kMaxExp;
     return __builtin_amdgcn_rcpf(1.0f + __expf(-x));
 }
+// (selects, no branches: the clipped cases are rare and a branch would end
+// the basic block the scheduler interleaves independent work in)
 __device__ __forceinline__ float grad_label1_r(float f, float alpha) {
-    if (f > kMaxExp) return 0.0f;
-    if (f < -kMaxExp) return alpha;
-    return (1.0f - sig_entry(f)) * alpha;
+    const float g = (1.0f - sig_entry(f)) * alpha;
+    return f > kMaxExp ? 0.0f : (f < -kMaxExp ? alpha : g);
 }
 __device__ __forceinline__ float grad_label0_r(float f, float alpha) {
-    if (f > kMaxExp) return -alpha;
-    if (f < -kMaxExp) return 0.0f;
-    return (0.0f - sig_entry(f)) * alpha;
+    const float g = (0.0f - sig_entry(f)) * alpha;
+    return f > kMaxExp ? -alpha : (f < -kMaxExp ? 0.0f : g);
 }
 
 __device__ __forceinline__ float softplusf(float x) {  // log(1 + e^x)

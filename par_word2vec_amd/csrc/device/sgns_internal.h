@@ -39,7 +39,7 @@ struct FastArgs {
     void *w_in, *w_out;
     int32_t ld, d, nq, W, R, NEG, B, V, kmax;
     int32_t lds_per_wave;         // bytes
-    int32_t lds_region;           // bytes of the ring / snapshot region at the start of a wave's LDS
+    int32_t lds_negs;             // byte offset of the batch's negatives in a wave's LDS (after the snapshot)
     int32_t persistent;
     int32_t rounding;             // bf16 write-back: 0 stochastic, 1 nearest
     int32_t rep_mfma;             // bf16 tables: repulsion products on the matrix cores
@@ -77,10 +77,6 @@ hipError_t launch_sample(const float *prob, const int32_t *alias, int V, uint64_
                          int32_t *out, hipStream_t st);
 hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int flags, hipStream_t st);
 
-// per-wave LDS bytes of the fast kernel
-// ring values + snapshots during attraction, reused as the batch-row
-// snapshot during repulsion; then negatives (drawn, unique, multiplicity)
-// and the three ring slot tables
 // Smallest power of two >= r (r >= 1): the position->slot table is indexed
 // by position & (pow2_ceil(R) - 1).
 __host__ __device__ inline int pow2_ceil(int r) {
@@ -102,23 +98,25 @@ constexpr int kMfTS = 20;
 __host__ __device__ inline int mf_snap_stride(int ld) { return ld + kMfPad; }
 inline int mf_bytes(int ld, int B) { return (B + 1) * mf_snap_stride(ld) * 2 + 16 * kMfTS * 2; }
 
-// The region a wave's LDS starts with: the attraction ring (fp32 value +
-// loaded value per pair and slot) and, reusing it, the repulsion snapshot.
-inline int fast_lds_region(int R, int nq, int bf16, int B, int ld, int mfma) {
+// A wave's LDS holds, during a sentence's attraction, the ring (fp32 value +
+// loaded value per pair and slot) and, once the ring has been flushed, the
+// repulsion state over the same bytes: the batch-row snapshot at offset 0,
+// then the batch's negatives (drawn, unique, multiplicities: 3 x kmax int32).
+// Offset of the negatives = the snapshot's size.
+inline int fast_lds_negs(int nq, int bf16, int B, int ld, int mfma) {
     const int e = bf16 ? 4 : 8;
-    // + one row of 64 pairs: lanes past the row read and rewrite entries
-    // beyond the last slot, which must stay inside this wave's region
-    int ring = R * nq * (8 + e) + 64 * 8;
     int snap = B * nq * e;
     if (mfma && mf_bytes(ld, B) > snap) snap = mf_bytes(ld, B);
-    const int region = ring > snap ? ring : snap;
-    return (region + 15) & ~15;
+    return (snap + 15) & ~15;
 }
 
 inline int fast_lds_per_wave(int R, int nq, int bf16, int kmax, int B, int ld, int mfma) {
-    const int region = fast_lds_region(R, nq, bf16, B, ld, mfma);
-    (void)R;
-    return (region + 3 * kmax * 4 + 15) & ~15;
+    const int e = bf16 ? 4 : 8;
+    // + one zero pair per lane after the ring: a lane whose last pair lies
+    // past the row reads and writes it in place of that pair
+    const int ring = R * nq * (8 + e) + 64 * 8;
+    const int rep = fast_lds_negs(nq, bf16, B, ld, mfma) + 3 * kmax * 4;
+    return ((ring > rep ? ring : rep) + 15) & ~15;
 }
 
 }  // namespace sgns
