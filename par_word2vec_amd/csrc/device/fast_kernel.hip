@@ -157,49 +157,42 @@ constexpr uint32_t kNoRow = 0xFFFFF000u;
 __device__ bf16x2_t buffer_atomic_pk_add_bf16(bf16x2_t v, __amdgpu_buffer_rsrc_t r, int off, int soff, int aux)
     __asm("llvm.amdgcn.raw.ptr.buffer.atomic.fadd.v2bf16");
 
-// Write back one lane's output dimensions jD..jD+3 of row `row` (D layout of a
-// 16x16 tile: lane r of a 16-lane row holds tile row r).  Lanes r and r^8
-// swap halves (DPP row_ror:8) so that the first atomic covers tile rows 0..7
-// with 32 contiguous bytes each and the second rows 8..15: one memory request
-// per row and instruction instead of two.  quad_steps() computes the two
-// packed steps and their byte offsets (called with every lane active, or
-// skipped for an empty tile); issue_quad() sends both atomics from every lane,
-// every time: a pair with nothing to add (no row, past d, a step that rounds
-// to zero, an empty tile) goes to kNoRow, which the range check drops without
-// a memory request.  Unconditional atomics keep the compiler's wait counts
-// exact, so a wait for a load never also waits for atomics issued after it.
-struct Quad {
-    uint32_t sa, sr;  // packed bf16 steps: tile rows 0..7, rows 8..15
-    uint32_t oa, orr; // their byte offsets (kNoRow: nothing to add)
-};
-__device__ __forceinline__ void issue_quad(__amdgpu_buffer_rsrc_t rw, const Quad &qd) {
-    buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, qd.sa), rw, (int)qd.oa, 0, 0);
-    buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, qd.sr), rw, (int)qd.orr, 0, 0);
+// A dropped atomic (offset kNoRow): issued where a loop's entry must have
+// the same operations in flight as its later iterations.
+__device__ __forceinline__ void drop_atomic(__amdgpu_buffer_rsrc_t rw) {
+    buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, 0u), rw, (int)kNoRow, 0, 0);
 }
-__device__ __forceinline__ int dpp_ror8_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false); }
-__device__ __forceinline__ Quad quad_steps(int row, uint32_t row_bytes, int jD, int d, uint32_t b01, uint32_t b23,
-                                           f32x4_t acc, bool lo, uint32_t &rs, int nearest) {
-    // lo lanes send their second pair, hi lanes their first
-    const float sx = lo ? acc[2] : acc[0], sy = lo ? acc[3] : acc[1];
-    const uint32_t sb = lo ? b23 : b01;
-    const float rx = __builtin_bit_cast(float, dpp_ror8_i(__builtin_bit_cast(int, sx)));
-    const float ry = __builtin_bit_cast(float, dpp_ror8_i(__builtin_bit_cast(int, sy)));
-    const uint32_t rb = (uint32_t)dpp_ror8_i((int)sb);
-    const int prow = dpp_ror8_i(row);
-    const int q = (jD >> 1) + (lo ? 0 : 1);  // lo lanes write the first pair, hi lanes the second
-    const bool inside = 2 * q < d;           // the pair starts below d
-    // rows 0..7 of the tile
-    const int ra = lo ? row : prow;
-    Quad qd;
-    qd.sa = bf16_step(Pair<uint16_t>::tof(lo ? b01 : rb), lo ? make_float2(acc[0], acc[1]) : make_float2(rx, ry), rs,
-                      nearest);
-    qd.oa = ra >= 0 && inside && qd.sa != 0u ? (uint32_t)ra * row_bytes + 4u * q : kNoRow;
-    // rows 8..15
-    const int rr = lo ? prow : row;
-    qd.sr = bf16_step(Pair<uint16_t>::tof(lo ? rb : b23), lo ? make_float2(rx, ry) : make_float2(acc[2], acc[3]), rs,
-                      nearest);
-    qd.orr = rr >= 0 && inside && qd.sr != 0u ? (uint32_t)rr * row_bytes + 4u * q : kNoRow;
-    return qd;
+
+// Write back a step pair of one 16 x 16 output tile whose lane (c, g) holds
+// column c (a dimension) of rows 4g..4g+3 (one per register): x = step jt
+// (dimensions jt*16 + c), y = step jt+1.  For each register, lanes c and c^1
+// exchange one value (DPP quad_perm [1,0,3,2]): an even lane then holds
+// dimensions (c, c+1) of step jt, an odd lane (c+15, c+16) of step jt+1 --
+// the pair starting at dimp -- so one packed-bf16 atomic writes four rows x
+// 64 contiguous bytes (the atomic unit's granularity: four requests per
+// instruction).  base = the values the rounding is decided against (the same
+// pair), off = the rows' byte offsets (kNoRow: no row).  Every lane issues all
+// four atomics; a pair with nothing to add (no row, at or past d, a step
+// that rounds to zero, an inactive tile) goes to kNoRow, which the range
+// check drops without a memory request, so the compiler's wait counts
+// include every atomic.
+__device__ __forceinline__ void write_pair(__amdgpu_buffer_rsrc_t rw, bool active, const uint32_t (&off)[4],
+                                           const uint32_t (&base)[4], f32x4_t x, f32x4_t y, int dimp, int d, int odd,
+                                           uint32_t &rs, int nearest) {
+    uint32_t st[4] = {0u, 0u, 0u, 0u}, o[4] = {kNoRow, kNoRow, kNoRow, kNoRow};
+    if (active) {  // uniform
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float sent = odd ? x[r] : y[r];
+            const float got = __builtin_bit_cast(
+                float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, sent), 0xB1, 0xF, 0xF, false));
+            const float2 dl = odd ? make_float2(got, y[r]) : make_float2(x[r], got);
+            st[r] = bf16_step(Pair<uint16_t>::tof(base[r]), dl, rs, nearest);
+            o[r] = off[r] != kNoRow && dimp < d && st[r] != 0u ? off[r] + 2u * (uint32_t)dimp : kNoRow;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, st[r]), rw, (int)o[r], 0, 0);
 }
 
 template <int NP>
@@ -306,12 +299,17 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 // keep the next tile's row prefetch (4*NP x 16 bytes per lane) out of this one
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // ---- dC^T = N^T Gm, sixteen dimensions at a time (N as read for G:
-            // no W_out row of the group has been written yet).  Vector memory
-            // operations of a wave retire in issue order, so the A operands of
-            // step jt+1 are loaded before step jt's write-back atomics: waiting
-            // for them then never waits for those atomics.
-            // byte offsets of this lane's A-operand rows (kNoRow past U)
+            // ---- dC = Gm^T N and dN = Gm C, sixteen dimensions a step, steps
+            // in pairs.  With Gm as the A operand the output tile has one
+            // dimension per lane and four rows per register, and one atomic
+            // per register writes four rows x 32 dimensions of a step pair
+            // (write_pair).  dC first: N as read for G (no W_out row of the
+            // group has been written yet).  Both loops issue a step pair's
+            // operand loads before its atomics and enter with the same
+            // operations in flight as every later pair (dropped atomics after
+            // the prologue loads), so each wait is for loads one pair old.
+            const int odd = r16 & 1;
+            // byte offsets of this lane's B-operand rows of N (kNoRow past U)
             uint32_t offA[kMfU][4];
 #pragma unroll
             for (int ut = 0; ut < kMfU; ut++) {
@@ -322,9 +320,8 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 }
             }
             // raw 16-bit values, one register each: packing them at load time
-            // would make the wave wait for the loads there
-            // (steps past the last read dimensions past d: the next row's or
-            // zeros, multiplied but never written back)
+            // would make the wave wait for the loads there (steps past the last
+            // read dimensions past d: the next row's or zeros, never written back)
             auto load_a = [&](int jt, uint32_t (&av)[kMfU][4]) {
                 const uint32_t jb = 2u * (uint32_t)(jt * 16 + r16);
 #pragma unroll
@@ -334,55 +331,61 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                         av[ut][r] = __builtin_amdgcn_raw_buffer_load_b16(rout, offA[ut][r] + jb, 0, kSc1);
                 }
             };
-            // steps in pairs (JT rounded up to even), operands two steps ahead
-            uint32_t aX[kMfU][4], aY[kMfU][4];
-            // (each followed by the atomics a step issues, all dropped: the loop is
-            // then entered with the operations in flight of every later step,
-            // and its waits are counted for that, not for a shorter prologue)
-            const Quad none{0u, 0u, kNoRow, kNoRow};
-            load_a(0, aX);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int it = 0; it < kMfI; it++) issue_quad(rin, none);
-            load_a(1, aY);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int it = 0; it < kMfI; it++) issue_quad(rin, none);
-            auto step_c = [&](int jt, uint32_t (&av)[kMfU][4]) {
-                const int jD = jt * 16 + 4 * g4;  // first of this lane's four output dimensions
-                s16x4_t ac[kMfU];
+            auto pack = [&](const uint32_t (&av)[kMfU][4], s16x4_t (&ac)[kMfU]) {
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++)
                     ac[ut] = s16x4_t{(short)av[ut][0], (short)av[ut][1], (short)av[ut][2], (short)av[ut][3]};
-                load_a(jt + 2, av);
-                __builtin_amdgcn_sched_barrier(0);
-                f32x4_t accC[kMfI];
+            };
+            // this lane's output rows of dC: positions it*16 + 4*g4 + r
+            uint32_t offC[kMfI][4];
 #pragma unroll
-                for (int it = 0; it < kMfI; it++) accC[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+            for (int it = 0; it < kMfI; it++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int ip = it * 16 + 4 * g4 + r;
+                    offC[it][r] = ip < ni ? (uint32_t)bat[ip] * row_bytes : kNoRow;
+                }
+            }
+            uint32_t aX[kMfU][4], aY[kMfU][4];
+            load_a(0, aX);
+            load_a(1, aY);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4 * kMfI; k++) drop_atomic(rin);
+            for (int jt = 0; jt < JT; jt += 2) {  // JT even: dimensions up to ld
+                s16x4_t acX[kMfU], acY[kMfU];
+                pack(aX, acX);
+                pack(aY, acY);
+                load_a(jt + 2, aX);
+                load_a(jt + 3, aY);
+                __builtin_amdgcn_sched_barrier(0);
+                f32x4_t cX[kMfI], cY[kMfI];
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) {
+                    cX[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                    cY[it] = cX[it];
+                }
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++) {
                     if (ut >= UT) continue;
 #pragma unroll
                     for (int it = 0; it < kMfI; it++) {
                         if (it >= IT) continue;
-                        accC[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ac[ut], gd[ut][it], accC[it], 0, 0, 0);
+                        cX[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], acX[ut], cX[it], 0, 0, 0);
+                        cY[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], acY[ut], cY[it], 0, 0, 0);
                     }
                 }
+                const int dimp = jt * 16 + (r16 & ~1) + (odd ? 16 : 0);  // this lane's pair of the step pair
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) {
-                    Quad qd{0u, 0u, kNoRow, kNoRow};
-                    if (it < IT) {  // uniform
-                        const uint2 b = *reinterpret_cast<const uint2 *>(snapC + srow(it * 16 + r16) * SC + jD);
-                        qd = quad_steps(rowC[it], row_bytes, jD, d, b.x, b.y, accC[it], r16 < 8, rs, a.rounding);
-                    }
-                    issue_quad(rin, qd);
+                    uint32_t base[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        base[r] = *reinterpret_cast<const uint32_t *>(snapC + srow(it * 16 + 4 * g4 + r) * SC + dimp);
+                    write_pair(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, d, odd, rs, a.rounding);
                 }
-            };
-            for (int jt = 0; jt < JT; jt += 2) {
-                step_c(jt, aX);
-                step_c(jt + 1, aY);
             }
-            // ---- Gm^T: each 16 x 16 tile transposed through LDS
+            // ---- Gm^T: each 16 x 16 tile transposed through LDS (the A operand of dN)
             s16x4_t gt[kMfU][kMfI];
 #pragma unroll
             for (int ut = 0; ut < kMfU; ut++) {
@@ -396,65 +399,66 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                     gt[ut][it] = __builtin_bit_cast(s16x4_t, t2);
                 }
             }
-            // ---- dN^T = C^T Gm^T, sixteen dimensions at a time; the current
-            // values the write-backs round against are fetched one step ahead
-            // (rowN = -1 past U; dimensions past d read the row's zero padding or
-            // the next row, never written back)
-            uint32_t offB[kMfU];
+            // ---- dN: the current values the write-backs round against are
+            // loaded one step pair ahead, one dword per output register
+            // this lane's output rows of dN: negatives ut*16 + 4*g4 + r (kNoRow past U)
+            uint32_t offN[kMfU][4];
 #pragma unroll
-            for (int ut = 0; ut < kMfU; ut++) offB[ut] = rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes : kNoRow;
-            auto load_base = [&](int jt, u32x2_t (&bv)[kMfU]) {
-                const uint32_t jb = 2u * (uint32_t)(jt * 16 + 4 * g4);
+            for (int ut = 0; ut < kMfU; ut++) {
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++)
-                    bv[ut] = __builtin_amdgcn_raw_buffer_load_b64(rout, offB[ut] + jb, 0, kSc1);
+                for (int r = 0; r < 4; r++) offN[ut][r] = offA[ut][r];  // the same rows
+            }
+            auto load_base = [&](int jt, uint32_t (&bv)[kMfU][4]) {
+                const uint32_t jb = 2u * (uint32_t)(jt * 16 + (r16 & ~1) + (odd ? 16 : 0));
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) bv[ut][r] = __builtin_amdgcn_raw_buffer_load_b32(rout, offN[ut][r] + jb, 0, kSc1);
+                }
             };
-            u32x2_t bX[kMfU], bY[kMfU];  // steps in pairs, two steps ahead
-            load_base(0, bX);  // (as for dC: each followed by a step's dropped atomics)
+            uint32_t bn[kMfU][4];
+            load_base(0, bn);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int ut = 0; ut < kMfU; ut++) issue_quad(rout, none);
-            load_base(1, bY);
-            __builtin_amdgcn_sched_barrier(0);
+            for (int k = 0; k < 4 * kMfU; k++) drop_atomic(rout);
+            for (int jt = 0; jt < JT; jt += 2) {
+                // this pair's values out of bn (explicit copies, so the loads
+                // below can reuse its registers), then the next pair's
+                uint32_t bcur[kMfU][4];
 #pragma unroll
-            for (int ut = 0; ut < kMfU; ut++) issue_quad(rout, none);
-            auto step_n = [&](int jt, u32x2_t (&bv)[kMfU]) {
-                const int jA = jt * 16 + r16;
-                const int jD = jt * 16 + 4 * g4;
-                // this step's values out of bv (explicit copies, so the loads
-                // below can reuse its registers), then the step two ahead
-                u32x2_t bcur[kMfU];
+                for (int ut = 0; ut < kMfU; ut++) {
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) bcur[ut] = u32x2_t{copy_u32(bv[ut].x), copy_u32(bv[ut].y)};
-                load_base(jt + 2, bv);
+                    for (int r = 0; r < 4; r++) bcur[ut][r] = copy_u32(bn[ut][r]);
+                }
+                load_base(jt + 2, bn);
                 __builtin_amdgcn_sched_barrier(0);
-                f32x4_t accN[kMfU];
+                f32x4_t nX[kMfU], nY[kMfU];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) accN[ut] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                for (int ut = 0; ut < kMfU; ut++) {
+                    nX[ut] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                    nY[ut] = nX[ut];
+                }
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) {
                     if (it >= IT) continue;
-                    s16x4_t cv;
+                    s16x4_t cvX, cvY;  // C: positions it*16 + 4*g4 + r at this lane's dimension of each step
 #pragma unroll
-                    for (int r = 0; r < 4; r++) cv[r] = (short)snapC[srow(it * 16 + 4 * g4 + r) * SC + jA];
+                    for (int r = 0; r < 4; r++) {
+                        const uint16_t *q = snapC + srow(it * 16 + 4 * g4 + r) * SC + jt * 16 + r16;
+                        cvX[r] = (short)q[0];
+                        cvY[r] = (short)q[16];
+                    }
 #pragma unroll
                     for (int ut = 0; ut < kMfU; ut++) {
                         if (ut >= UT) continue;
-                        accN[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(cv, gt[ut][it], accN[ut], 0, 0, 0);
+                        nX[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvX, nX[ut], 0, 0, 0);
+                        nY[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvY, nY[ut], 0, 0, 0);
                     }
                 }
+                const int dimp = jt * 16 + (r16 & ~1) + (odd ? 16 : 0);
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-                    Quad qd{0u, 0u, kNoRow, kNoRow};
-                    if (ut < UT)  // uniform
-                        qd = quad_steps(rowN[ut], row_bytes, jD, d, bcur[ut].x, bcur[ut].y, accN[ut], r16 < 8, rs,
-                                        a.rounding);
-                    issue_quad(rout, qd);
-                }
-            };
-            for (int jt = 0; jt < JT; jt += 2) {
-                step_n(jt, bX);
-                step_n(jt + 1, bY);
+                for (int ut = 0; ut < kMfU; ut++)
+                    write_pair(rout, ut < UT, offN[ut], bcur[ut], nX[ut], nY[ut], dimp, d, odd, rs, a.rounding);
             }
         }
     }
