@@ -36,7 +36,8 @@ enum {
 
 enum par_sgns_precision { PAR_SGNS_FP32 = 0, PAR_SGNS_BF16 = 1 };
 /* How a bf16 table entry absorbs an fp32 update: stochastic rounding (the
- * default; unbiased for updates below half an ulp) or round-to-nearest. */
+ * default; unbiased for updates below half an ulp) or round-to-nearest
+ * (ties away from zero). */
 enum par_sgns_rounding { PAR_SGNS_ROUND_STOCHASTIC = 0, PAR_SGNS_ROUND_NEAREST = 1 };
 
 /* How plain/persistent modes evaluate a repulsion batch (the three SGEMMs

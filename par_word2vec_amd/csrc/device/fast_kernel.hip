@@ -56,17 +56,17 @@ __device__ __forceinline__ uint32_t bf16_step(float2 base, float2 dlt, uint32_t 
     rs ^= rs << 13;  // per-lane xorshift32 stream (Marsaglia 2003) for rounding noise
     rs ^= rs >> 17;
     rs ^= rs << 5;
-    const uint32_t noise = rs;
-    const float x = base.x + dlt.x, y = base.y + dlt.y;
-    // both roundings add a value below the kept 16 bits and truncate: half an
-    // ulp (+1 on ties to even) for nearest, 16 uniform bits for stochastic
-    const uint32_t bx = __builtin_bit_cast(uint32_t, x), by = __builtin_bit_cast(uint32_t, y);
-    const uint32_t ax = nearest ? 0x7FFFu + ((bx >> 16) & 1u) : (noise & 0xFFFFu);
-    const uint32_t ay = nearest ? 0x7FFFu + ((by >> 16) & 1u) : (noise >> 16);
-    const uint32_t rx = (bx + ax) >> 16, ry = (by + ay) >> 16;
-    const float sx = __builtin_bit_cast(float, rx << 16) - base.x;
-    const float sy = __builtin_bit_cast(float, ry << 16) - base.y;
-    return bf16_rne(sx) | (bf16_rne(sy) << 16);  // exact: a multiple of base's ulp
+    // both roundings add a value below the kept 16 bits to the fp32 bit
+    // pattern and truncate: 16 uniform bits for stochastic, half an ulp for
+    // nearest (ties away from zero)
+    const uint32_t noise = nearest ? 0x80008000u : rs;
+    const uint32_t bx = __builtin_bit_cast(uint32_t, base.x + dlt.x), by = __builtin_bit_cast(uint32_t, base.y + dlt.y);
+    const float sx = __builtin_bit_cast(float, (bx + (noise & 0xFFFFu)) & 0xFFFF0000u) - base.x;
+    const float sy = __builtin_bit_cast(float, (by + (noise >> 16)) & 0xFFFF0000u) - base.y;
+    // the steps' high halves: exact when base and the rounded value share a
+    // binade (s - base is then a multiple of base's ulp below 2^8 ulps);
+    // otherwise the step's low bits are truncated toward zero
+    return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, sy), __builtin_bit_cast(uint32_t, sx), 0x07060302u);
 }
 __device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 base, float2 dlt,
                                         uint32_t &rs, int nearest) {
@@ -172,12 +172,12 @@ __device__ __forceinline__ void drop_atomic(__amdgpu_buffer_rsrc_t rw) {
 // 64 contiguous bytes (the atomic unit's granularity: four requests per
 // instruction).  base = the values the rounding is decided against (the same
 // pair), off = the rows' byte offsets (kNoRow: no row).  Every lane issues all
-// four atomics; a pair with nothing to add (no row, at or past d, a step
-// that rounds to zero, an inactive tile) goes to kNoRow, which the range
+// four atomics; a pair with nothing to add (a step that rounds to zero, no
+// row, an inactive tile) lands past the resource's size, which the range
 // check drops without a memory request, so the compiler's wait counts
 // include every atomic.
 __device__ __forceinline__ void write_pair(__amdgpu_buffer_rsrc_t rw, bool active, const uint32_t (&off)[4],
-                                           const uint32_t (&base)[4], f32x4_t x, f32x4_t y, int dimp, int d, int odd,
+                                           const uint32_t (&base)[4], f32x4_t x, f32x4_t y, int dimp, int odd,
                                            uint32_t &rs, int nearest) {
     uint32_t st[4] = {0u, 0u, 0u, 0u}, o[4] = {kNoRow, kNoRow, kNoRow, kNoRow};
     if (active) {  // uniform
@@ -188,7 +188,9 @@ __device__ __forceinline__ void write_pair(__amdgpu_buffer_rsrc_t rw, bool activ
                 float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, sent), 0xB1, 0xF, 0xF, false));
             const float2 dl = odd ? make_float2(got, y[r]) : make_float2(x[r], got);
             st[r] = bf16_step(Pair<uint16_t>::tof(base[r]), dl, rs, nearest);
-            o[r] = off[r] != kNoRow && dimp < d && st[r] != 0u ? off[r] + 2u * (uint32_t)dimp : kNoRow;
+            // (kNoRow + 2 dimp is past the resource too; pairs at or past d
+            // hold zero padding and a zero update, so their step is 0)
+            o[r] = st[r] != 0u ? off[r] + 2u * (uint32_t)dimp : kNoRow;
         }
     }
 #pragma unroll
@@ -382,7 +384,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                         base[r] = *reinterpret_cast<const uint32_t *>(snapC + srow(it * 16 + 4 * g4 + r) * SC + dimp);
-                    write_pair(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, d, odd, rs, a.rounding);
+                    write_pair(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, odd, rs, a.rounding);
                 }
             }
             // ---- Gm^T: each 16 x 16 tile transposed through LDS (the A operand of dN)
@@ -458,7 +460,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 const int dimp = jt * 16 + (r16 & ~1) + (odd ? 16 : 0);
 #pragma unroll
                 for (int ut = 0; ut < kMfU; ut++)
-                    write_pair(rout, ut < UT, offN[ut], bcur[ut], nX[ut], nY[ut], dimp, d, odd, rs, a.rounding);
+                    write_pair(rout, ut < UT, offN[ut], bcur[ut], nX[ut], nY[ut], dimp, odd, rs, a.rounding);
             }
         }
     }
@@ -692,76 +694,66 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                     for (int t = 0; t < NP; t++) u[t] = *ring_at(slot, t);
                 };
                 auto dot = [&](const float2 (&x)[NP], const float2 (&y)[NP]) {
-                    float s = 0.0f;
+                    float s = x[0].x * y[0].x + x[0].y * y[0].y;
 #pragma unroll
-                    for (int t = 0; t < NP; t++) s += x[t].x * y[t].x + x[t].y * y[t].y;
+                    for (int t = 1; t < NP; t++) s += x[t].x * y[t].x + x[t].y * y[t].y;
                     return s;
                 };
+                // Pair k: on entry v = v_k, cur = u_k and f = f_k.  The next
+                // pair's row goes into nxt; the two row buffers alternate over
+                // an unrolled step pair, so no row is ever copied.
+                float2 uA[NP], uB[NP];
                 int slot = slot_of(pos(0));
-                float2 u[NP], up[NP];
-                read_row(slot, u);
-#pragma unroll
-                for (int t = 0; t < NP; t++) up[t] = make_float2(0.0f, 0.0f);
-                float A = wave_sum(dot(u, v)), B = 0.0f, gp = 0.0f;
-                // one pair; `more`: pair k+1 exists and its row is read here
-                auto step = [&](int k, auto more) {
-#pragma unroll
-                    for (int t = 0; t < NP; t++) {  // v_k
-                        v[t].x += gp * up[t].x;
-                        v[t].y += gp * up[t].y;
-                    }
-                    const float f = A + gp * B;
+                read_row(slot, uA);
+                float f = wave_sum(dot(uA, v));
+                auto step = [&](int k, float2 (&cur)[NP], float2 (&nxt)[NP], auto more, auto loss) {
                     int slot_n = slot;
-                    float2 un[NP];
                     float An = 0.0f, Bn = 0.0f;
-                    if constexpr (decltype(more)::value) {
+                    if constexpr (decltype(more)::value) {  // u_{k+1}.v_k and u_{k+1}.u_k
                         slot_n = slot_of(pos(k + 1));
-                        read_row(slot_n, un);
-                        An = wave_sum(dot(un, v));
-                        Bn = wave_sum(dot(un, u));
+                        read_row(slot_n, nxt);
+                        An = wave_sum(dot(nxt, v));
+                        Bn = wave_sum(dot(nxt, cur));
                     }
                     const float g = grad_label1_r(f, alpha_p);
                     float2 nw[NP];
 #pragma unroll
                     for (int t = 0; t < NP; t++) {
-                        nw[t] = make_float2(u[t].x + g * v[t].x, u[t].y + g * v[t].y);
+                        nw[t] = make_float2(cur[t].x + g * v[t].x, cur[t].y + g * v[t].y);
                         *ring_at(slot, t) = nw[t];
                     }
-                    fbuf = lane == k ? f : fbuf;  // logit of pair k, kept in lane k
+#pragma unroll
+                    for (int t = 0; t < NP; t++) {  // v_{k+1}
+                        v[t].x += g * cur[t].x;
+                        v[t].y += g * cur[t].y;
+                    }
+                    if constexpr (decltype(loss)::value) fbuf = lane == k ? f : fbuf;  // pair k's logit in lane k
                     if constexpr (decltype(more)::value) {
                         if (slot_n == slot) {  // pair k+1 reads the row pair k just wrote
-                            float2 vn[NP];
 #pragma unroll
-                            for (int t = 0; t < NP; t++) {
-                                un[t] = nw[t];
-                                vn[t] = make_float2(v[t].x + g * u[t].x, v[t].y + g * u[t].y);
-                            }
-                            An = wave_sum(dot(un, vn));
-                            Bn = 0.0f;
+                            for (int t = 0; t < NP; t++) nxt[t] = nw[t];
+                            f = wave_sum(dot(nxt, v));
+                        } else {
+                            f = An + g * Bn;
                         }
-#pragma unroll
-                        for (int t = 0; t < NP; t++) {
-                            up[t] = u[t];
-                            u[t] = un[t];
-                        }
-                        A = An;
-                        B = Bn;
                         slot = slot_n;
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < NP; t++) {
-                            up[t] = u[t];
-                        }
                     }
-                    gp = g;
                 };
-                for (int k = 0; k + 1 < npair; k++) step(k, std::true_type{});
-                step(npair - 1, std::false_type{});
-#pragma unroll
-                for (int t = 0; t < NP; t++) {  // the last pair's W_out update
-                    v[t].x += gp * up[t].x;
-                    v[t].y += gp * up[t].y;
-                }
+                auto pairs = [&](auto loss) {
+                    int k = 0;
+                    for (; k + 2 < npair; k += 2) {
+                        step(k, uA, uB, std::true_type{}, loss);
+                        step(k + 1, uB, uA, std::true_type{}, loss);
+                    }
+                    if (k + 1 < npair) {
+                        step(k, uA, uB, std::true_type{}, loss);
+                        step(k + 1, uB, uA, std::false_type{}, loss);
+                    } else {
+                        step(k, uA, uB, std::false_type{}, loss);
+                    }
+                };
+                if (a.compute_loss) pairs(std::true_type{});
+                else pairs(std::false_type{});
             }
             cnt.pairs += (uint32_t)npair;
             if (a.compute_loss && lane < npair) loss_a += (double)softplusf(-fbuf);  // this lane's share

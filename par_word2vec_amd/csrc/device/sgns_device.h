@@ -31,18 +31,26 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
 // Sum over the 64 lanes, result uniform.  Butterfly inside each row of 16
 // lanes (quad_perm xor1, xor2, row_half_mirror, row_mirror), then gfx9's row
 // broadcasts: row_bcast:15 adds lane 15 of rows 0/2 into rows 1/3, row_bcast:31
-// adds lane 31 into rows 2/3, and lane 63 holds (r3 + r2) + (r1 + r0).
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_rows(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, false));
+// adds lane 31 into rows 2/3, and lane 63 holds (r3 + r2) + (r1 + r0).  The
+// two broadcast steps are single DPP adds whose masked-out rows are left
+// unwritten (the compiler emits a DPP move, a move of the old value and an
+// add for each); the s_nop gives the two wait states a DPP read needs after
+// a VALU write of its source.
+__device__ __forceinline__ float add_row_bcast15(float v) {
+    asm("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ float add_row_bcast31(float v) {
+    asm("s_nop 1\n\tv_add_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(v));
+    return v;
 }
 __device__ __forceinline__ float wave_sum(float v) {
     v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
     v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
     v += dpp<0x141>(v);  // row_half_mirror
     v += dpp<0x140>(v);  // row_mirror
-    v += dpp_rows<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-    v += dpp_rows<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    v = add_row_bcast15(v);  // rows 1 and 3
+    v = add_row_bcast31(v);  // rows 2 and 3
     return readlane_f(v, 63);
 }
 
