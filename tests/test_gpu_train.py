@@ -91,6 +91,8 @@ def test_det_zipf1m_full(zipf1m):
     ((300, "fp32"), 5, 10, 24, 1e-3),
     ((128, "fp32"), 8, 5, 16, 1e-4),
     ((100, "fp32"), 5, 0, 24, 1e-3),
+    ((75, "fp32"), 4, 6, 20, 1e-3),    # odd dimension: a pair straddles the row's end
+    ((200, "fp32"), 5, 5, 24, 1e-3),   # two pairs per lane, lanes 36..63 past the row
 ])
 def test_hogwild_kernel_exact_schedule(precision_dim, window, negative, batch, sample):
     """The Hogwild kernel, one wavefront, driven by the oracle's schedule:
@@ -154,6 +156,7 @@ def _rms(x):
     (128, 8, 5, 32, False),   # the largest batch of the matrix path
     (64, 8, 25, 24, False),   # up to 375 negatives: several negative groups
     (300, 5, 10, 24, True),   # one-word sentences: repulsion only
+    (75, 4, 6, 20, False),    # odd dimension: the last written pair straddles d
 ])
 def test_matrix_repulsion_exact_schedule(dim, window, negative, batch, single):
     """bf16 tables, round-to-nearest write-backs, the oracle's exact schedule
