@@ -119,13 +119,14 @@ __device__ __forceinline__ int alias_draw(const float *prob, const int32_t *alia
 //   G    = N C^T                v_mfma_f32_16x16x32_bf16 (bf16 operands are
 //                               the table values, products exact, fp32 sums)
 //   Gm   = m (0 - sigmoid(G)) alpha   (exp-table sigmoid, cpu:641-645), bf16
-//   dC^T = N^T Gm               v_mfma_f32_16x16x16_bf16
-//   dN^T = C^T Gm^T             v_mfma_f32_16x16x16_bf16
+//   dC   = Gm^T N               v_mfma_f32_16x16x16_bf16
+//   dN   = Gm C                 v_mfma_f32_16x16x16_bf16
 // the three cblas_sgemm calls of cpu:633-656, from one snapshot.  An output
-// tile holds four consecutive dimensions of one row per lane (D layout of a
+// tile of dC or dN holds one dimension of four rows per lane (D layout of a
 // 16x16 MFMA: lane l, register r -> row 4(l/16)+r, column l%16), written back
-// as two packed bf16 pairs: each row of C takes one write-back per group of
-// kMfU*16 unique negatives (one per batch up to 80), each row of N one.
+// a step pair at a time (write_pair): each row of C takes one write-back per
+// group of kMfU*16 unique negatives (one per batch up to 64), each row of N
+// one.
 // The batch holds at most kMfI*16 = 32 positions (the host keeps B <= 32 on
 // this path), so every product reads the one snapshot the reference uses.
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));

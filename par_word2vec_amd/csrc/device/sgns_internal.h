@@ -91,7 +91,7 @@ __host__ __device__ inline int pow2_ceil(int r) {
 // the batch read; the pad spreads the four row groups of a transposed read
 // over distinct banks) and one 16 x 16 bf16 tile for transposing the gradient
 // (rows of kMfTS).
-constexpr int kMfU = 5;
+constexpr int kMfU = 4;
 constexpr int kMfI = 2;
 constexpr int kMfPad = 4;
 constexpr int kMfTS = 20;
