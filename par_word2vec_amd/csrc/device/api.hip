@@ -451,9 +451,9 @@ constexpr int64_t kMinSentencesPerWave = 16;
 // Resident blocks per CU for a block of wpb waves: 160 KB of LDS per CU (the
 // kernel has no per-block LDS of its own); at most kMaxWavesCu waves per CU
 // (two per SIMD at the kernel's register budget).
-constexpr int kLdsCu = 160 * 1024, kBlockStatic = 0, kMaxWavesCu = 8, kMaxWpb = 8;
+constexpr int kLdsCu = 160 * 1024, kMaxWavesCu = 8, kMaxWpb = 8;
 static int blocks_per_cu(int lds_per_wave, int wpb) {
-    int b = kLdsCu / (lds_per_wave * wpb + kBlockStatic);
+    int b = kLdsCu / (lds_per_wave * wpb);
     if (b * wpb > kMaxWavesCu) b = kMaxWavesCu / wpb;
     return b;
 }
@@ -532,7 +532,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
             f.plan_b = h->p_b; f.plan_alpha = h->p_alpha; f.plan_batch_off = h->p_batch_off;
             f.plan_neg_off = h->p_neg_off; f.plan_negs = h->p_negs;
         }
-        f.exp_table = h->exp_table; f.alias_prob = h->alias_prob; f.alias_idx = h->alias_idx;
+        f.alias_prob = h->alias_prob; f.alias_idx = h->alias_idx;
         f.w_in = h->w_in; f.w_out = h->w_out;
         f.ld = h->ld; f.d = h->cfg.dim; f.nq = h->nq; f.W = h->cfg.window; f.R = h->R; f.NEG = h->cfg.negative;
         f.B = h->cfg.batch_size; f.V = h->cfg.vocab_size; f.kmax = h->kmax;

@@ -198,6 +198,23 @@ __device__ __forceinline__ void write_pair(__amdgpu_buffer_rsrc_t rw, bool activ
     for (int r = 0; r < 4; r++) buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, st[r]), rw, (int)o[r], 0, 0);
 }
 
+// One row's write-back on a matrix-core instance (bf16 tables below 4 GiB -
+// 4 KiB, addressed through a buffer resource): every lane issues its NP
+// atomics and a pair past the row (q >= nq) or with a zero step lands past
+// the resource, dropped by the range check -- no branch per pair.
+template <int NP>
+__device__ __forceinline__ void store_delta_rsrc(__amdgpu_buffer_rsrc_t rw, uint32_t row, uint32_t row_bytes, int nq,
+                                                 int lane, const float2 (&base)[NP], const float2 (&dlt)[NP],
+                                                 uint32_t &rs, int nearest) {
+#pragma unroll
+    for (int t = 0; t < NP; t++) {
+        const int q = lane + kWave * t;
+        const uint32_t st = bf16_step(base[t], dlt[t], rs, nearest);
+        const uint32_t o = q < nq && st != 0u ? row * row_bytes + 4u * (uint32_t)q : kNoRow;
+        buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, st), rw, (int)o, 0, 0);
+    }
+}
+
 template <int NP>
 __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t *bat, int isz, int U,
                                                const int32_t *uneg, const int32_t *umul, char *lds, int lane,
@@ -487,6 +504,12 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     T *w_in = static_cast<T *>(a.w_in);
     T *w_out = static_cast<T *>(a.w_out);
     const int ld = a.ld, nq = a.nq, W = a.W, R = a.R;
+    // matrix-core instances address the tables through buffer resources
+    // (the host enables them only below 4 GiB - 4 KiB)
+    const uint32_t row_bytes = (uint32_t)ld * (uint32_t)sizeof(T);
+    const uint32_t tab_bytes = (uint32_t)a.V * row_bytes;
+    const __amdgpu_buffer_rsrc_t rin_w = __builtin_amdgcn_make_buffer_rsrc(a.w_in, (short)0, (int)tab_bytes, kRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rout_w = __builtin_amdgcn_make_buffer_rsrc(a.w_out, (short)0, (int)tab_bytes, kRsrcWord3);
     // the host picks NP = ceil(nq / 64): only a lane's last pair can be past the row
     __builtin_assume(nq > kWave * (NP - 1) && nq <= kWave * NP);
     const int64_t o0 = a.sent_off[s];
@@ -556,6 +579,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
 #pragma unroll
         for (int t = 0; t < NP; t++) {
             const int q = lane + kWave * t;
+            u[t] = u0[t] = make_float2(0.0f, 0.0f);
             if (t < NP - 1 || q < nq) {
                 u[t] = ring_u[slot * nq + q];
                 u0[t] = Pair<T>::tof(ring_0[slot * nq + q]);
@@ -563,11 +587,18 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
         }
     };
     auto write_slot = [&](int64_t row, const float2 (&u)[NP], const float2 (&u0)[NP]) {
+        if constexpr (MF) {
+            float2 du[NP];
 #pragma unroll
-        for (int t = 0; t < NP; t++) {
-            const int q = lane + kWave * t;
-            if (t < NP - 1 || q < nq)
-                wb_pair(w_in, row, ld, q, u0[t], make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y), rs, a.rounding);
+            for (int t = 0; t < NP; t++) du[t] = make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y);
+            store_delta_rsrc<NP>(rin_w, (uint32_t)row, row_bytes, nq, lane, u0, du, rs, a.rounding);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NP; t++) {
+                const int q = lane + kWave * t;
+                if (t < NP - 1 || q < nq)
+                    wb_pair(w_in, row, ld, q, u0[t], make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y), rs, a.rounding);
+            }
         }
     };
     auto flush = [&](int slot) {
@@ -782,7 +813,8 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 float2 dv[NP];
 #pragma unroll
                 for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
-                store_delta<T, NP>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
+                if constexpr (MF) store_delta_rsrc<NP>(rout_w, (uint32_t)wp, row_bytes, nq, lane, v0, dv, rs, a.rounding);
+                else store_delta<T, NP>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
             }
             if (leave) write_slot(frow, fu, fu0);
             carried = carry_next;

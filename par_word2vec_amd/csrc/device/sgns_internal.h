@@ -33,7 +33,6 @@ struct FastArgs {
     const int64_t *sent_raw;
     const int32_t *order;         // persistent queue order (may be null)
     const int64_t *queue_raw;     // raw words dequeued before each queue slot (with order)
-    const float *exp_table;
     const float *alias_prob;
     const int32_t *alias_idx;
     void *w_in, *w_out;
