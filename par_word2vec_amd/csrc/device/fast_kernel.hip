@@ -8,22 +8,23 @@
 // Attraction (cpu:529-568): the W_in rows of the sliding window [p-W, p+W]
 // live in an LDS ring of R = 2W+1 word-keyed slots holding the current fp32
 // value and the value that was loaded; a row is fetched once when its word
-// enters the window (prefetched one centre ahead) and written back once when
+// enters the window (loaded two centres ahead) and written back once when
 // its last occurrence leaves, as delta = current - loaded.  The centre's
 // W_out row stays in registers for all its pairs (and across a repeated
-// centre).  Pairs are applied sequentially, exactly as the reference does, so
-// later pairs see earlier updates.
+// centre).  Pairs are applied in the reference's order with its in-place
+// semantics, so later pairs see earlier updates.
 //
 // Repulsion (cpu:571-679): per batch of B positions, K = rep_b * negative
 // shared negatives are drawn from the on-device alias table with the
 // reference's in-sentence rejection; the K x B logits, the label-0 gradient
-// and both updates are computed in register blocks of NB negatives.
+// and both updates come from one snapshot, on the matrix cores (bf16 tables,
+// repulsion_mfma) or in register blocks of NB negatives (vector path).
 //
 // Write-back: fp32 tables take the delta with global float atomics (no lost
 // updates, as the reference GPU's atomicAdd, text.cu:438-439, 710-717);
-// bf16 tables re-read the current value, add the fp32 delta and store it
-// stochastically rounded (a Hogwild read-modify-write, as the reference
-// CPU's plain += at cpu:561-565).
+// bf16 tables add the packed bf16 step that moves the wave's base value to
+// the stochastically rounded base + delta (packed-bf16 atomics: a concurrent
+// update is kept, never overwritten).
 //
 // Randomness: Philox4x32-10 keyed by the seed; the counter names the
 // sentence, epoch, purpose and index of each draw.
