@@ -230,6 +230,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
         __builtin_amdgcn_make_buffer_rsrc(w_in, (short)0, (int)((uint32_t)a.V * row_bytes), kRsrcWord3);
     uint16_t *snapC = reinterpret_cast<uint16_t *>(lds);
     uint16_t *tr = snapC + (a.B + 1) * SC;
+    uint16_t *stg = tr + 16 * kMfTS;  // two staging tiles of 16 x kMfSS
     const int r16 = lane & 15, g4 = lane >> 4;
     const int KS = ld / 32, JT = (d + 15) / 16;
     const int ni = isz, IT = (ni + 15) >> 4;
@@ -320,92 +321,6 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                 // keep the next tile's row prefetch (4*NP x 16 bytes per lane) out of this one
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // ---- dC = Gm^T N and dN = Gm C, sixteen dimensions a step, steps
-            // in pairs.  With Gm as the A operand the output tile has one
-            // dimension per lane and four rows per register, and one atomic
-            // per register writes four rows x 32 dimensions of a step pair
-            // (write_pair).  dC first: N as read for G (no W_out row of the
-            // group has been written yet).  Both loops issue a step pair's
-            // operand loads before its atomics and enter with the same
-            // operations in flight as every later pair (dropped atomics after
-            // the prologue loads), so each wait is for loads one pair old.
-            const int odd = r16 & 1;
-            // byte offsets of this lane's B-operand rows of N (kNoRow past U)
-            uint32_t offA[kMfU][4];
-#pragma unroll
-            for (int ut = 0; ut < kMfU; ut++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int ua = u0 + ut * 16 + 4 * g4 + r;
-                    offA[ut][r] = ut < UT && ua < U ? (uint32_t)uneg[ua] * row_bytes : kNoRow;
-                }
-            }
-            // raw 16-bit values, one register each: packing them at load time
-            // would make the wave wait for the loads there (steps past the last
-            // read dimensions past d: the next row's or zeros, never written back)
-            auto load_a = [&](int jt, uint32_t (&av)[kMfU][4]) {
-                const uint32_t jb = 2u * (uint32_t)(jt * 16 + r16);
-#pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        av[ut][r] = __builtin_amdgcn_raw_buffer_load_b16(rout, offA[ut][r] + jb, 0, kSc1);
-                }
-            };
-            auto pack = [&](const uint32_t (&av)[kMfU][4], s16x4_t (&ac)[kMfU]) {
-#pragma unroll
-                for (int ut = 0; ut < kMfU; ut++)
-                    ac[ut] = s16x4_t{(short)av[ut][0], (short)av[ut][1], (short)av[ut][2], (short)av[ut][3]};
-            };
-            // this lane's output rows of dC: positions it*16 + 4*g4 + r
-            uint32_t offC[kMfI][4];
-#pragma unroll
-            for (int it = 0; it < kMfI; it++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int ip = it * 16 + 4 * g4 + r;
-                    offC[it][r] = ip < ni ? (uint32_t)bat[ip] * row_bytes : kNoRow;
-                }
-            }
-            uint32_t aX[kMfU][4], aY[kMfU][4];
-            load_a(0, aX);
-            load_a(1, aY);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int k = 0; k < 4 * kMfI; k++) drop_atomic(rin);
-            for (int jt = 0; jt < JT; jt += 2) {  // JT even: dimensions up to ld
-                s16x4_t acX[kMfU], acY[kMfU];
-                pack(aX, acX);
-                pack(aY, acY);
-                load_a(jt + 2, aX);
-                load_a(jt + 3, aY);
-                __builtin_amdgcn_sched_barrier(0);
-                f32x4_t cX[kMfI], cY[kMfI];
-#pragma unroll
-                for (int it = 0; it < kMfI; it++) {
-                    cX[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-                    cY[it] = cX[it];
-                }
-#pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-                    if (ut >= UT) continue;
-#pragma unroll
-                    for (int it = 0; it < kMfI; it++) {
-                        if (it >= IT) continue;
-                        cX[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], acX[ut], cX[it], 0, 0, 0);
-                        cY[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], acY[ut], cY[it], 0, 0, 0);
-                    }
-                }
-                const int dimp = jt * 16 + (r16 & ~1) + (odd ? 16 : 0);  // this lane's pair of the step pair
-#pragma unroll
-                for (int it = 0; it < kMfI; it++) {
-                    uint32_t base[4];
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        base[r] = *reinterpret_cast<const uint32_t *>(snapC + srow(it * 16 + 4 * g4 + r) * SC + dimp);
-                    write_pair(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, odd, rs, a.rounding);
-                }
-            }
             // ---- Gm^T: each 16 x 16 tile transposed through LDS (the A operand of dN)
             s16x4_t gt[kMfU][kMfI];
 #pragma unroll
@@ -420,66 +335,113 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                     gt[ut][it] = __builtin_bit_cast(s16x4_t, t2);
                 }
             }
-            // ---- dN: the current values the write-backs round against are
-            // loaded one step pair ahead, one dword per output register
-            // this lane's output rows of dN: negatives ut*16 + 4*g4 + r (kNoRow past U)
-            uint32_t offN[kMfU][4];
+            // ---- dC = Gm^T N and dN = Gm C, one pass over step pairs (32
+            // dimensions).  With Gm^T / Gm as the A operand an output tile holds
+            // one dimension per lane and four rows per register, and one atomic
+            // per register writes four rows x 32 dimensions (write_pair).  N
+            // enters as one 128-bit load per lane and tile per step pair (16
+            // rows x 32 dimensions, one pair ahead), staged through LDS to give
+            // dC its B operands and dN its rounding bases.  A pair's slices are
+            // loaded before that pair's write-backs, so dC and the bases see N
+            // as G did (no dimension of a W_out row of the group is written
+            // before it is read).  Every loop iteration issues the same loads
+            // and atomics (dropped ones included) and is entered with the same
+            // operations in flight, so each wait is for the previous pair's loads.
+            const int odd = r16 & 1;
+            uint32_t offT[kMfU];      // this lane's row of tile ut (row r16), its 8-dimension chunk g4
+            uint32_t offN[kMfU][4];   // dN output rows: negatives ut*16 + 4*g4 + r (kNoRow past U)
 #pragma unroll
             for (int ut = 0; ut < kMfU; ut++) {
+                offT[ut] = (rowN[ut] >= 0 ? (uint32_t)rowN[ut] * row_bytes : kNoRow) + 16u * (uint32_t)g4;
 #pragma unroll
-                for (int r = 0; r < 4; r++) offN[ut][r] = offA[ut][r];  // the same rows
+                for (int r = 0; r < 4; r++) {
+                    const int ua = u0 + ut * 16 + 4 * g4 + r;
+                    offN[ut][r] = ut < UT && ua < U ? (uint32_t)uneg[ua] * row_bytes : kNoRow;
+                }
             }
-            auto load_base = [&](int jt, uint32_t (&bv)[kMfU][4]) {
-                const uint32_t jb = 2u * (uint32_t)(jt * 16 + (r16 & ~1) + (odd ? 16 : 0));
+            uint32_t offC[kMfI][4];   // dC output rows: positions it*16 + 4*g4 + r
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
+            for (int it = 0; it < kMfI; it++) {
 #pragma unroll
-                    for (int r = 0; r < 4; r++) bv[ut][r] = __builtin_amdgcn_raw_buffer_load_b32(rout, offN[ut][r] + jb, 0, kSc1);
+                for (int r = 0; r < 4; r++) {
+                    const int ip = it * 16 + 4 * g4 + r;
+                    offC[it][r] = ip < ni ? (uint32_t)bat[ip] * row_bytes : kNoRow;
                 }
+            }
+            // (slices past the last pair read the next row's start or zeros, unused)
+            auto load_slice = [&](int jt, u32x4_t (&sl)[kMfU]) {
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++)
+                    sl[ut] = __builtin_amdgcn_raw_buffer_load_b128(rout, offT[ut] + 32u * (uint32_t)jt, 0, kSc1);
             };
-            uint32_t bn[kMfU][4];
-            load_base(0, bn);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int k = 0; k < 4 * kMfU; k++) drop_atomic(rout);
-            for (int jt = 0; jt < JT; jt += 2) {
-                // this pair's values out of bn (explicit copies, so the loads
-                // below can reuse its registers), then the next pair's
-                uint32_t bcur[kMfU][4];
-#pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-#pragma unroll
-                    for (int r = 0; r < 4; r++) bcur[ut][r] = copy_u32(bn[ut][r]);
-                }
-                load_base(jt + 2, bn);
+            const int dpair = (r16 & ~1) + (odd ? 16 : 0);  // first dimension of this lane's pair in a step pair
+            auto step_pair = [&](int jt, const u32x4_t (&sl)[kMfU], u32x4_t (&nxt)[kMfU]) {
+                load_slice(jt + 2, nxt);
                 __builtin_amdgcn_sched_barrier(0);
-                f32x4_t nX[kMfU], nY[kMfU];
-#pragma unroll
-                for (int ut = 0; ut < kMfU; ut++) {
-                    nX[ut] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-                    nY[ut] = nX[ut];
-                }
+                // C: positions it*16 + 4*g4 + r at dimensions jt*16 + r16 and +16
+                s16x4_t cvX[kMfI], cvY[kMfI];
 #pragma unroll
                 for (int it = 0; it < kMfI; it++) {
-                    if (it >= IT) continue;
-                    s16x4_t cvX, cvY;  // C: positions it*16 + 4*g4 + r at this lane's dimension of each step
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const uint16_t *q = snapC + srow(it * 16 + 4 * g4 + r) * SC + jt * 16 + r16;
-                        cvX[r] = (short)q[0];
-                        cvY[r] = (short)q[16];
-                    }
-#pragma unroll
-                    for (int ut = 0; ut < kMfU; ut++) {
-                        if (ut >= UT) continue;
-                        nX[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvX, nX[ut], 0, 0, 0);
-                        nY[ut] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvY, nY[ut], 0, 0, 0);
+                        cvX[it][r] = (short)q[0];
+                        cvY[it][r] = (short)q[16];
                     }
                 }
-                const int dimp = jt * 16 + (r16 & ~1) + (odd ? 16 : 0);
+                f32x4_t cX[kMfI], cY[kMfI];
 #pragma unroll
-                for (int ut = 0; ut < kMfU; ut++)
-                    write_pair(rout, ut < UT, offN[ut], bcur[ut], nX[ut], nY[ut], dimp, odd, rs, a.rounding);
+                for (int it = 0; it < kMfI; it++) cX[it] = cY[it] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+                const int dimp = jt * 16 + dpair;
+#pragma unroll
+                for (int ut = 0; ut < kMfU; ut++) {
+                    uint16_t *st = stg + (ut & 1) * 16 * kMfSS;
+                    // stage this lane's 8 dimensions of row r16
+                    uint32_t *sw = reinterpret_cast<uint32_t *>(st + r16 * kMfSS + 8 * g4);
+                    *reinterpret_cast<uint2 *>(sw) = make_uint2(sl[ut].x, sl[ut].y);
+                    *reinterpret_cast<uint2 *>(sw + 2) = make_uint2(sl[ut].z, sl[ut].w);
+                    // dC's B operands (rows 4*g4 + r at dimensions r16 and 16 + r16)
+                    // and dN's rounding bases (the pair this lane writes)
+                    s16x4_t bX, bY;
+                    uint32_t base[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const uint16_t *row = st + (4 * g4 + r) * kMfSS;
+                        bX[r] = (short)row[r16];
+                        bY[r] = (short)row[16 + r16];
+                        base[r] = *reinterpret_cast<const uint32_t *>(row + dpair);
+                    }
+                    f32x4_t nX = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, nY = nX;
+                    if (ut < UT) {  // uniform
+#pragma unroll
+                        for (int it = 0; it < kMfI; it++) {
+                            if (it >= IT) continue;
+                            cX[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], bX, cX[it], 0, 0, 0);
+                            cY[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], bY, cY[it], 0, 0, 0);
+                            nX = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvX[it], nX, 0, 0, 0);
+                            nY = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvY[it], nY, 0, 0, 0);
+                        }
+                    }
+                    write_pair(rout, ut < UT, offN[ut], base, nX, nY, dimp, odd, rs, a.rounding);
+                }
+                // dC write-back, rounding against the snapshot
+#pragma unroll
+                for (int it = 0; it < kMfI; it++) {
+                    uint32_t base[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        base[r] = *reinterpret_cast<const uint32_t *>(snapC + srow(it * 16 + 4 * g4 + r) * SC + dimp);
+                    write_pair(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, odd, rs, a.rounding);
+                }
+            };
+            u32x4_t sX[kMfU], sY[kMfU];
+            load_slice(0, sX);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4 * (kMfU + kMfI); k++) drop_atomic(rout);  // a step pair's atomics
+            for (int jt = 0; jt < JT; jt += 4) {  // pairs of step pairs: dimensions up to ld
+                step_pair(jt, sX, sY);
+                step_pair(jt + 2, sY, sX);
             }
         }
     }

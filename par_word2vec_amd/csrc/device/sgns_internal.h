@@ -88,14 +88,18 @@ __host__ __device__ inline int pow2_ceil(int r) {
 // is processed in groups of kMfU*16 unique negatives.  Its LDS holds the batch
 // snapshot (B rows of ld + kMfPad bf16 and one zero row that tile rows past
 // the batch read; the pad spreads the four row groups of a transposed read
-// over distinct banks) and one 16 x 16 bf16 tile for transposing the gradient
-// (rows of kMfTS).
+// over distinct banks), one 16 x 16 bf16 tile for transposing the gradient
+// (rows of kMfTS) and two staging tiles of 16 rows x 32 dimensions of N
+// (rows of kMfSS: the four row groups of a column read fall on distinct banks).
 constexpr int kMfU = 4;
 constexpr int kMfI = 2;
 constexpr int kMfPad = 4;
 constexpr int kMfTS = 20;
+constexpr int kMfSS = 36;
 __host__ __device__ inline int mf_snap_stride(int ld) { return ld + kMfPad; }
-inline int mf_bytes(int ld, int B) { return (B + 1) * mf_snap_stride(ld) * 2 + 16 * kMfTS * 2; }
+inline int mf_bytes(int ld, int B) {
+    return (B + 1) * mf_snap_stride(ld) * 2 + 16 * kMfTS * 2 + 2 * 16 * kMfSS * 2;
+}
 
 // A wave's LDS holds, during a sentence's attraction, the ring (fp32 value +
 // loaded value per pair and slot) and, once the ring has been flushed, the
