@@ -45,8 +45,7 @@ struct par_sgns_handle_s {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void *w_in = nullptr, *w_out = nullptr;
     float *exp_table = nullptr;
-    float *alias_prob = nullptr;
-    int32_t *alias_idx = nullptr;
+    uint2 *alias_tab = nullptr;  // per bucket: {keep probability (float bits), alias}
     bool have_exp = false, have_sampler = false, have_weights = false;
     // fast-mode corpus
     int32_t *tokens = nullptr;
@@ -203,7 +202,7 @@ int par_sgns_destroy(par_sgns_handle h) {
     (void)hipSetDevice(h->cfg.device);
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-    dfree(h->w_in); dfree(h->w_out); dfree(h->exp_table); dfree(h->alias_prob); dfree(h->alias_idx);
+    dfree(h->w_in); dfree(h->w_out); dfree(h->exp_table); dfree(h->alias_tab);
     dfree(h->tokens); dfree(h->sent_off); dfree(h->sent_raw); dfree(h->order); dfree(h->queue_raw);
     dfree(h->p_tokens); dfree(h->p_negs); dfree(h->p_b); dfree(h->p_sent_off); dfree(h->p_batch_off);
     dfree(h->p_neg_off); dfree(h->p_alpha); dfree(h->det_scratch);
@@ -270,9 +269,14 @@ int par_sgns_set_sampler(par_sgns_handle h, const int64_t *entries, int32_t V) {
     // a word with zero mass must never be returned by its own bucket
     if (prob[0] > 0.0f && w[0] == 0.0) prob[0] = 0.0f;
     if (alias[0] == 0) alias[0] = 1;
-    int rc = upload(h, h->alias_prob, prob.data(), (size_t)V);
-    if (rc) return rc;
-    rc = upload(h, h->alias_idx, alias.data(), (size_t)V);
+    // one 8-byte entry per bucket: a draw is one load
+    std::vector<uint2> tab((size_t)V);
+    for (int32_t i = 0; i < V; i++) {
+        uint32_t pb;
+        std::memcpy(&pb, &prob[(size_t)i], 4);
+        tab[(size_t)i] = make_uint2(pb, (uint32_t)alias[(size_t)i]);
+    }
+    int rc = upload(h, h->alias_tab, tab.data(), (size_t)V);
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->st));
     h->have_sampler = true;
@@ -287,7 +291,7 @@ int par_sgns_sample_negatives(par_sgns_handle h, uint64_t seed, int64_t sent, in
     int32_t *d = nullptr;
     int rc = dalloc(h, d, (size_t)n);
     if (rc) return rc;
-    hipError_t e = sgns::launch_sample(h->alias_prob, h->alias_idx, h->cfg.vocab_size, seed, sent, n, d, h->st);
+    hipError_t e = sgns::launch_sample(h->alias_tab, h->cfg.vocab_size, seed, sent, n, d, h->st);
     if (e == hipSuccess) e = hipMemcpyAsync(out, d, (size_t)n * 4, hipMemcpyDeviceToHost, h->st);
     if (e == hipSuccess) e = hipStreamSynchronize(h->st);
     dfree(d);
@@ -532,7 +536,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
             f.plan_b = h->p_b; f.plan_alpha = h->p_alpha; f.plan_batch_off = h->p_batch_off;
             f.plan_neg_off = h->p_neg_off; f.plan_negs = h->p_negs;
         }
-        f.alias_prob = h->alias_prob; f.alias_idx = h->alias_idx;
+        f.alias = h->alias_tab;
         f.w_in = h->w_in; f.w_out = h->w_out;
         f.ld = h->ld; f.d = h->cfg.dim; f.nq = h->nq; f.W = h->cfg.window; f.R = h->R; f.NEG = h->cfg.negative;
         f.B = h->cfg.batch_size; f.V = h->cfg.vocab_size; f.kmax = h->kmax;

@@ -107,10 +107,12 @@ __device__ __forceinline__ void store_delta(T *w, int64_t row, int ld, int nq, i
 }
 
 // One draw from the unigram^0.75 alias table (before in-sentence rejection).
-__device__ __forceinline__ int alias_draw(const float *prob, const int32_t *alias, int V, U4 rr) {
+// A bucket's entry is {keep probability (float bits), alias}: one 8-byte load.
+__device__ __forceinline__ int alias_draw(const uint2 *tab, int V, U4 rr) {
     const uint32_t bucket = __umulhi(rr.x, (uint32_t)V);
     const float u2 = (float)(rr.y >> 8) * (1.0f / 16777216.0f);
-    return u2 < prob[bucket] ? (int)bucket : alias[bucket];
+    const uint2 e = tab[bucket];
+    return u2 < __builtin_bit_cast(float, e.x) ? (int)bucket : (int)e.y;
 }
 
 // ------------------------------------------- repulsion on the matrix cores
@@ -842,7 +844,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 continue;
             }
             const uint32_t idx = (uint32_t)bi * 8192u + (uint32_t)k;
-            int cand = alias_draw(a.alias_prob, a.alias_idx, a.V, draw(a.seed, s, ep, kTagNeg, idx));
+            int cand = alias_draw(a.alias, a.V, draw(a.seed, s, ep, kTagNeg, idx));
             // Redraw uniformly over [1, V-1] while the candidate occurs in the
             // sentence.  The reference loops without bound (cpu:611-615); the
             // bound here only matters when the sentence holds nearly the whole
@@ -1041,10 +1043,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void s
 
 // n draws of the sampler for sentence id `sent` (the counter layout of the
 // trainer's negatives), for testing the on-device distribution.
-__global__ void sgns_sample_kernel(const float *prob, const int32_t *alias, int V, uint64_t seed, int64_t sent,
-                                   int64_t n, int32_t *out) {
+__global__ void sgns_sample_kernel(const uint2 *alias, int V, uint64_t seed, int64_t sent, int64_t n, int32_t *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = alias_draw(prob, alias, V, draw(seed, sent + (i >> 13), 0, kTagNeg, (uint32_t)(i & 8191)));
+    if (i < n) out[i] = alias_draw(alias, V, draw(seed, sent + (i >> 13), 0, kTagNeg, (uint32_t)(i & 8191)));
 }
 
 __global__ void sgns_reset_kernel(unsigned long long *counters, int n, double *loss, unsigned long long *queue) {
@@ -1100,11 +1101,11 @@ hipError_t launch_fast(const FastArgs &a, int bf16, int np, int wpb, int blocks,
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_sample(const float *prob, const int32_t *alias, int V, uint64_t seed, int64_t sent, int64_t n,
-                         int32_t *out, hipStream_t st) {
+hipError_t launch_sample(const uint2 *alias, int V, uint64_t seed, int64_t sent, int64_t n, int32_t *out,
+                         hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sgns_sample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, prob, alias, V, seed,
-                       sent, n, out);
+    hipLaunchKernelGGL(sgns_sample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, alias, V, seed, sent,
+                       n, out);
     return hipGetLastError();
 }
 

@@ -33,8 +33,7 @@ struct FastArgs {
     const int64_t *sent_raw;
     const int32_t *order;         // persistent queue order (may be null)
     const int64_t *queue_raw;     // raw words dequeued before each queue slot (with order)
-    const float *alias_prob;
-    const int32_t *alias_idx;
+    const uint2 *alias;           // alias table: {keep probability (float bits), alias} per bucket
     void *w_in, *w_out;
     int32_t ld, d, nq, W, R, NEG, B, V, kmax;
     int32_t lds_per_wave;         // bytes
@@ -72,8 +71,8 @@ hipError_t launch_det(const DetArgs &a, hipStream_t st);
 hipError_t launch_fast(const FastArgs &a, int bf16, int np, int waves_per_block, int blocks, hipStream_t st);
 hipError_t launch_reset(unsigned long long *counters, int n_counters, double *loss, unsigned long long *queue,
                         hipStream_t st);
-hipError_t launch_sample(const float *prob, const int32_t *alias, int V, uint64_t seed, int64_t sent, int64_t n,
-                         int32_t *out, hipStream_t st);
+hipError_t launch_sample(const uint2 *alias, int V, uint64_t seed, int64_t sent, int64_t n, int32_t *out,
+                         hipStream_t st);
 hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int flags, hipStream_t st);
 
 // Smallest power of two >= r (r >= 1): the position->slot table is indexed
