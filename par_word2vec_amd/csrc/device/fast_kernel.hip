@@ -690,10 +690,14 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
 #pragma unroll
                     for (int t = 0; t < NP; t++) u[t] = *ring_at(slot, t);
                 };
-                auto dot = [&](const float2 (&x)[NP], const float2 (&y)[NP]) {
-                    float s = x[0].x * y[0].x + x[0].y * y[0].y;
+                auto dot = [&](const float2 (&x)[NP], const float2 (&y)[NP]) {  // one FMA chain
+                    float s = x[0].x * y[0].x;
+                    s = __builtin_fmaf(x[0].y, y[0].y, s);
 #pragma unroll
-                    for (int t = 1; t < NP; t++) s += x[t].x * y[t].x + x[t].y * y[t].y;
+                    for (int t = 1; t < NP; t++) {
+                        s = __builtin_fmaf(x[t].x, y[t].x, s);
+                        s = __builtin_fmaf(x[t].y, y[t].y, s);
+                    }
                     return s;
                 };
                 // Pair k: on entry v = v_k, cur = u_k and f = f_k.  The next
@@ -709,8 +713,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                     if constexpr (decltype(more)::value) {  // u_{k+1}.v_k and u_{k+1}.u_k
                         slot_n = slot_of(pos(k + 1));
                         read_row(slot_n, nxt);
-                        An = wave_sum(dot(nxt, v));
-                        Bn = wave_sum(dot(nxt, cur));
+                        wave_sum2(dot(nxt, v), dot(nxt, cur), An, Bn);
                     }
                     const float g = grad_label1_r(f, alpha_p);
                     float2 nw[NP];

@@ -53,6 +53,25 @@ __device__ __forceinline__ float wave_sum(float v) {
     v = add_row_bcast31(v);  // rows 2 and 3
     return readlane_f(v, 63);
 }
+// Two sums over the 64 lanes in one tree.  v_permlane32_swap trades the upper
+// half of a with the lower half of b, so after one add lanes 0-31 hold a's
+// half-sums and lanes 32-63 b's; the same row butterfly then reduces both,
+// and one row_bcast:15 step leaves a's total in row 1 and b's in row 3.
+// (The swap's two results are read into scalars first: clang's bit_cast of an
+// ext-vector element reads element 0 whatever the index.)
+__device__ __forceinline__ void wave_sum2(float a, float b, float &sa, float &sb) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                                    false, false);
+    const uint32_t lo = r[0], hi = r[1];
+    float v = __builtin_bit_cast(float, lo) + __builtin_bit_cast(float, hi);
+    v += dpp<0xB1>(v);
+    v += dpp<0x4E>(v);
+    v += dpp<0x141>(v);
+    v += dpp<0x140>(v);
+    v = add_row_bcast15(v);
+    sa = readlane_f(v, 31);
+    sb = readlane_f(v, 63);
+}
 
 __device__ __forceinline__ int64_t readfirst_i64(int64_t v) {
     int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)v);
@@ -169,23 +188,29 @@ __device__ __forceinline__ float grad_label0(float f, float alpha, const float *
 
 // The same rule with the table entry computed in registers instead of read
 // from LDS: entry i is e/(e+1) = 1/(1 + exp(-x_i)) at x_i = (i/1000*2-1)*6
-// (cpu:822-826), here with the hardware exp and reciprocal, which can differ
-// from the table's bits in the last places.  Used by the Hogwild kernel,
-// where the lookup's LDS round trip sits in every pair's dependent chain.
-__device__ __forceinline__ float sig_entry(float f) {
-    f = fminf(fmaxf(f, -(float)kMaxExp), (float)kMaxExp);  // callers clip outside [-6, 6]
-    const int i = (int)((f + (float)kMaxExp) * (1000.0f / 12.0f));
-    const float x = (float)i * (2.0f * kMaxExp / kExpTable) - (float)kMaxExp;
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-x));
+// (cpu:822-826).  The label-1 gradient takes 1 - sigma(x_i) = 1/(1 + e^x_i)
+// directly (no cancellation near x = 6), each exponential is one exp2 of
+// i*(0.012 log2 e) -/+ 6 log2 e, so results can differ from the table's
+// bits in the last places.  Used by the Hogwild kernels, where the lookup's
+// LDS round trip sits in every pair's dependent chain.  The index needs no
+// clamp: outside [-6, 6] the select discards it (and exp2 of a large
+// argument is inf or 0, never a trap).
+__device__ __forceinline__ float sig_index(float f) {  // (float)(int) of the table index, f in [-6, 6]
+    return __builtin_truncf((f + (float)kMaxExp) * (1000.0f / 12.0f));
 }
+constexpr float kLog2e = 1.44269504088896341f;
+constexpr float kGridStep2 = (2.0f * kMaxExp / kExpTable) * kLog2e;  // grid step, in log2 units
+constexpr float kGridLo2 = kMaxExp * kLog2e;                          // 6 log2 e
 // (selects, no branches: the clipped cases are rare and a branch would end
 // the basic block the scheduler interleaves independent work in)
 __device__ __forceinline__ float grad_label1_r(float f, float alpha) {
-    const float g = (1.0f - sig_entry(f)) * alpha;
+    const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(sig_index(f), kGridStep2, -kGridLo2));  // e^x_i
+    const float g = alpha * __builtin_amdgcn_rcpf(1.0f + e);
     return f > kMaxExp ? 0.0f : (f < -kMaxExp ? alpha : g);
 }
 __device__ __forceinline__ float grad_label0_r(float f, float alpha) {
-    const float g = (0.0f - sig_entry(f)) * alpha;
+    const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(sig_index(f), -kGridStep2, kGridLo2));  // e^-x_i
+    const float g = -alpha * __builtin_amdgcn_rcpf(1.0f + e);
     return f > kMaxExp ? -alpha : (f < -kMaxExp ? 0.0f : g);
 }
 
