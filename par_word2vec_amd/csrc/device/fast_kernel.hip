@@ -52,8 +52,9 @@ __device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, fl
     if (dlt.x != 0.0f) unsafeAtomicAdd(p, dlt.x);
     if (dlt.y != 0.0f) unsafeAtomicAdd(p + 1, dlt.y);
 }
-// The packed bf16 step s - base (0 if the pair does not move).
-__device__ __forceinline__ uint32_t bf16_step(float2 base, float2 dlt, uint32_t &rs, int nearest) {
+// The packed bf16 step s - base from base to s = cur rounded to bf16 (0 if
+// the pair does not move).
+__device__ __forceinline__ uint32_t bf16_step_to(float2 base, float2 cur, uint32_t &rs, int nearest) {
     rs ^= rs << 13;  // per-lane xorshift32 stream (Marsaglia 2003) for rounding noise
     rs ^= rs >> 17;
     rs ^= rs << 5;
@@ -61,13 +62,17 @@ __device__ __forceinline__ uint32_t bf16_step(float2 base, float2 dlt, uint32_t 
     // pattern and truncate: 16 uniform bits for stochastic, half an ulp for
     // nearest (ties away from zero)
     const uint32_t noise = nearest ? 0x80008000u : rs;
-    const uint32_t bx = __builtin_bit_cast(uint32_t, base.x + dlt.x), by = __builtin_bit_cast(uint32_t, base.y + dlt.y);
+    const uint32_t bx = __builtin_bit_cast(uint32_t, cur.x), by = __builtin_bit_cast(uint32_t, cur.y);
     const float sx = __builtin_bit_cast(float, (bx + (noise & 0xFFFFu)) & 0xFFFF0000u) - base.x;
     const float sy = __builtin_bit_cast(float, (by + (noise >> 16)) & 0xFFFF0000u) - base.y;
     // the steps' high halves: exact when base and the rounded value share a
     // binade (s - base is then a multiple of base's ulp below 2^8 ulps);
     // otherwise the step's low bits are truncated toward zero
     return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, sy), __builtin_bit_cast(uint32_t, sx), 0x07060302u);
+}
+// The same step for base + dlt.
+__device__ __forceinline__ uint32_t bf16_step(float2 base, float2 dlt, uint32_t &rs, int nearest) {
+    return bf16_step_to(base, make_float2(base.x + dlt.x, base.y + dlt.y), rs, nearest);
 }
 __device__ __forceinline__ void wb_pair(uint16_t *w, int64_t row, int ld, int q, float2 base, float2 dlt,
                                         uint32_t &rs, int nearest) {
@@ -202,17 +207,19 @@ __device__ __forceinline__ void write_pair(__amdgpu_buffer_rsrc_t rw, bool activ
 }
 
 // One row's write-back on a matrix-core instance (bf16 tables below 4 GiB -
-// 4 KiB, addressed through a buffer resource): every lane issues its NP
-// atomics and a pair past the row (q >= nq) or with a zero step lands past
-// the resource, dropped by the range check -- no branch per pair.
+// 4 KiB, addressed through a buffer resource): the step from the loaded
+// values `base` to the current values `cur` rounded to bf16.  Every lane
+// issues its NP atomics and a pair past the row (q >= nq) or with a zero
+// step lands past the resource, dropped by the range check -- no branch per
+// pair.
 template <int NP>
-__device__ __forceinline__ void store_delta_rsrc(__amdgpu_buffer_rsrc_t rw, uint32_t row, uint32_t row_bytes, int nq,
-                                                 int lane, const float2 (&base)[NP], const float2 (&dlt)[NP],
+__device__ __forceinline__ void store_value_rsrc(__amdgpu_buffer_rsrc_t rw, uint32_t row, uint32_t row_bytes, int nq,
+                                                 int lane, const float2 (&base)[NP], const float2 (&cur)[NP],
                                                  uint32_t &rs, int nearest) {
 #pragma unroll
     for (int t = 0; t < NP; t++) {
         const int q = lane + kWave * t;
-        const uint32_t st = bf16_step(base[t], dlt[t], rs, nearest);
+        const uint32_t st = bf16_step_to(base[t], cur[t], rs, nearest);
         const uint32_t o = q < nq && st != 0u ? row * row_bytes + 4u * (uint32_t)q : kNoRow;
         buffer_atomic_pk_add_bf16(__builtin_bit_cast(bf16x2_t, st), rw, (int)o, 0, 0);
     }
@@ -553,10 +560,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     };
     auto write_slot = [&](int64_t row, const float2 (&u)[NP], const float2 (&u0)[NP]) {
         if constexpr (MF) {
-            float2 du[NP];
-#pragma unroll
-            for (int t = 0; t < NP; t++) du[t] = make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y);
-            store_delta_rsrc<NP>(rin_w, (uint32_t)row, row_bytes, nq, lane, u0, du, rs, a.rounding);
+            store_value_rsrc<NP>(rin_w, (uint32_t)row, row_bytes, nq, lane, u0, u, rs, a.rounding);
         } else {
 #pragma unroll
             for (int t = 0; t < NP; t++) {
@@ -778,11 +782,14 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             for (int t = 0; t < NP; t++) vq0[t] = Pair<T>::copy(vq1[t]);
             // this centre's write-backs
             if (!carry_next) {
-                float2 dv[NP];
+                if constexpr (MF) {
+                    store_value_rsrc<NP>(rout_w, (uint32_t)wp, row_bytes, nq, lane, v0, v, rs, a.rounding);
+                } else {
+                    float2 dv[NP];
 #pragma unroll
-                for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
-                if constexpr (MF) store_delta_rsrc<NP>(rout_w, (uint32_t)wp, row_bytes, nq, lane, v0, dv, rs, a.rounding);
-                else store_delta<T, NP>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
+                    for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
+                    store_delta<T, NP>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
+                }
             }
             if (leave) write_slot(frow, fu, fu0);
             carried = carry_next;
