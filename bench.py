@@ -32,7 +32,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-from par_word2vec_amd import corpora, host as H  # noqa: E402
+from par_word2vec_amd import autotune, corpora, host as H  # noqa: E402
 from par_word2vec_amd.device import SgnsDevice  # noqa: E402
 from par_word2vec_amd.trainer import chunk_bounds  # noqa: E402
 
@@ -52,7 +52,8 @@ def algorithmic_bytes(tokens: int, negatives: int, dim: int, elem: int) -> float
 
 
 def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int, steps: int, csr, seed: int,
-                profile_only: bool = False, repulsion: str = "auto") -> dict:
+                profile_only: bool = False, repulsion: str = "auto", workload: str = "1b",
+                tune: bool = False) -> dict:
     w = c.workload
     V = len(c.vocab)
     dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision)
@@ -65,6 +66,20 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     upload_s = time.perf_counter() - t_up
     n = len(chunks)
     use_graph = mode == "persistent"
+    launch = None
+    if mode == "persistent":
+        # launch shape: the saved tuning of this workload (no trials in this
+        # process), or, with --autotune, the tuner run here on chunk 0
+        if tune:
+            b0, e0 = chunks[0]
+            res = dev.autotune(b0, e0, w.alpha, c.train_words, iters=w.iters, seed=seed, repulsion=repulsion)
+            launch = dict(res["best"], source="autotuned in this run")
+        else:
+            saved = autotune.load(workload, w, precision, repulsion)
+            if saved is not None:
+                launch = dict(autotune.apply(dev, saved),
+                              source=str(autotune.path_for(workload, precision, repulsion).relative_to(ROOT)))
+        log(f"{mode}/{precision}: launch shape {launch or 'built-in default'}")
 
     def step(i: int, prepare_only: bool = False) -> dict:
         b, e = chunks[i % n]
@@ -103,6 +118,8 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
                negatives=negs, pairs=sum(s["pairs"] for s in stats), mwords_s=raw / wall / 1e6,
                kernel_mwords_s=raw / (kern_ms / 1e3) / 1e6, alg_bytes=byts, launches=len(stats),
                avg_launch_ms=kern_ms / max(len(stats), 1), achieved_gbs=byts / (kern_ms / 1e3) / 1e9)
+    if mode == "persistent":
+        out["launch"] = launch or {"source": "built-in default (most resident waves, one sentence per dequeue)"}
     if not profile_only:
         g_in, _ = dev.get_weights()
         out["w_in"] = g_in
@@ -183,6 +200,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-score", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the launch-shape tuner in this process instead of using the saved tuning")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -209,7 +228,8 @@ def main() -> int:
     for v in args.variants.split(","):
         prec, mode, *rep = v.split("-")  # precision-mode[-repulsion]
         results[v] = run_variant(c, mode, prec, chunks, raw_per_chunk, args.warmup, steps, csr, args.seed,
-                                 profile_only=args.no_score, repulsion=rep[0] if rep else "auto")
+                                 profile_only=args.no_score, repulsion=rep[0] if rep else "auto",
+                                 workload=args.workload, tune=args.autotune)
     scores = {}
     if not args.no_score:
         for v, r in results.items():

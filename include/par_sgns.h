@@ -82,6 +82,9 @@ typedef struct par_sgns_train_args {
     uint64_t seed;        /* Philox key for windows, negatives, rounding */
     int64_t sent_begin;   /* sentence range [begin, end) of the uploaded corpus */
     int64_t sent_end;     /* (deterministic mode: of the uploaded plan) */
+    int32_t sentences_per_dequeue; /* persistent mode: queue slots a wavefront takes per
+                                      dequeue, 1..64; 0 = tuned default (1 untuned) */
+    int32_t reserved;
 } par_sgns_train_args;
 
 typedef struct par_sgns_stats {
@@ -147,6 +150,38 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *args, par_sgns_
  * use_graph, the captured hipGraph.  Lets a caller keep host work out of a
  * timed region. */
 int par_sgns_prepare(par_sgns_handle h, const par_sgns_train_args *args);
+
+/* One measured launch shape of the persistent kernel. */
+typedef struct par_sgns_tune_point {
+    int32_t waves_per_block;
+    int32_t waves_per_cu;          /* resident wavefronts per CU (grid = CUs * this / block) */
+    int32_t sentences_per_dequeue;
+    int32_t reserved;
+    double kernel_ms;              /* median over the repetitions */
+} par_sgns_tune_point;
+
+/* Launch-shape autotuner (no counterpart in the reference, whose launch shape
+ * is fixed at gamma*56 blocks of `dim` threads, text.cu:848-868, 899).  Times
+ * the persistent kernel on the sentence range of `args` (mode must be
+ * PAR_SGNS_PERSISTENT) for every feasible combination of waves per block
+ * {1, 2, 4, 8}, resident waves per CU {4, 6, 8} (bounded by the LDS a wave
+ * needs) and sentences per dequeue {1, 2, 4}, `reps` runs each (median).  Both
+ * tables are copied aside first and restored before every run and at the end,
+ * so tuning trains nothing.  The fastest shape becomes the handle's default
+ * for later calls that leave waves_per_block, blocks and
+ * sentences_per_dequeue at 0.  Every measurement goes to points[0..cap) in
+ * trial order, their number to *n_points (may exceed cap); the chosen shape to
+ * *best.  Needs device memory for one copy of both tables. */
+int par_sgns_autotune(par_sgns_handle h, const par_sgns_train_args *args, int32_t reps, par_sgns_tune_point *points,
+                      int32_t cap, int32_t *n_points, par_sgns_tune_point *best);
+/* Sets the handle's default persistent launch shape to a saved tuning (what
+ * par_sgns_autotune would choose; no counterpart in the reference): waves per
+ * block 1..8, resident waves per CU 1..8 (a multiple of the block; capped at
+ * what the LDS holds), sentences per dequeue 1..64.  All three 0 restore the
+ * built-in default (the block with the most resident waves, one sentence per
+ * dequeue). */
+int par_sgns_set_launch_shape(par_sgns_handle h, int32_t waves_per_block, int32_t waves_per_cu,
+                              int32_t sentences_per_dequeue);
 
 /* SGNS window-batch loss and its adjoint (no counterpart in the reference,
  * which fuses them into the update at cpu:551-565, 633-656).  Window w has

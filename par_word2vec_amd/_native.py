@@ -38,7 +38,16 @@ class ParSgnsTrainArgs(C.Structure):
     _fields_ = [("mode", i32), ("iter", i32), ("epoch", i32), ("compute_loss", i32),
                 ("alpha", f32), ("waves_per_block", i32), ("blocks", i32), ("use_graph", i32),
                 ("schedule", i32), ("repulsion", i32),
-                ("train_words", u64), ("seed", u64), ("sent_begin", i64), ("sent_end", i64)]
+                ("train_words", u64), ("seed", u64), ("sent_begin", i64), ("sent_end", i64),
+                ("sentences_per_dequeue", i32), ("reserved", i32)]
+
+
+class ParSgnsTunePoint(C.Structure):
+    _fields_ = [("waves_per_block", i32), ("waves_per_cu", i32), ("sentences_per_dequeue", i32),
+                ("reserved", i32), ("kernel_ms", f64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class ParSgnsStats(C.Structure):
@@ -100,6 +109,8 @@ SGNS_SYMBOLS = {
     "par_sgns_set_plan": (C.c_int, [vp, vp, vp, i64, vp, vp, i64, vp, vp, i64, vp, i64]),
     "par_sgns_train": (C.c_int, [vp, vp, vp]),
     "par_sgns_prepare": (C.c_int, [vp, vp]),
+    "par_sgns_autotune": (C.c_int, [vp, vp, i32, vp, i32, vp, vp]),
+    "par_sgns_set_launch_shape": (C.c_int, [vp, i32, i32, i32]),
     "par_sgns_loss_forward": (C.c_int, [vp, vp, vp, vp, i64, i32, i32, i32, vp]),
     "par_sgns_loss_adjoint": (C.c_int, [vp, vp, vp, vp, i64, i32, i32, i32, vp]),
     "par_sgns_get_grads": (C.c_int, [vp, vp, vp]),

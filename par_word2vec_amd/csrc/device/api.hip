@@ -26,12 +26,12 @@ namespace {
 
 struct GraphKey {
     int64_t b, e;
-    int32_t epoch, loss, wpb, blocks, iter;
+    int32_t epoch, loss, wpb, blocks, iter, grab;
     uint32_t alpha_bits;
     uint64_t seed, T;
     bool operator<(const GraphKey &o) const {
-        return std::tie(b, e, epoch, loss, wpb, blocks, iter, alpha_bits, seed, T) <
-               std::tie(o.b, o.e, o.epoch, o.loss, o.wpb, o.blocks, o.iter, o.alpha_bits, o.seed, o.T);
+        return std::tie(b, e, epoch, loss, wpb, blocks, iter, grab, alpha_bits, seed, T) <
+               std::tie(o.b, o.e, o.epoch, o.loss, o.wpb, o.blocks, o.iter, o.grab, o.alpha_bits, o.seed, o.T);
     }
 };
 
@@ -41,6 +41,8 @@ struct par_sgns_handle_s {
     par_sgns_config cfg{};
     int ld = 0, nq = 0, np = 0, bf16 = 0, kmax = 0, R = 0;
     int cu_count = 0;
+    // launch shape chosen by par_sgns_autotune (0: not tuned)
+    int tuned_wpb = 0, tuned_wpcu = 0, tuned_grab = 0;
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void *w_in = nullptr, *w_out = nullptr;
@@ -463,6 +465,18 @@ static int blocks_per_cu(int lds_per_wave, int wpb) {
 }
 
 static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, int &wpb, int &blocks) {
+    if (wpb <= 0 && h->tuned_wpb > 0) wpb = h->tuned_wpb;
+    if (blocks <= 0 && h->tuned_wpcu > 0) {  // the tuned resident waves per CU, where they fit
+        const int w = wpb > 0 ? wpb : 1;
+        int bpc = h->tuned_wpcu / w;
+        const int fit = blocks_per_cu(lds_per_wave, w);
+        if (bpc > fit) bpc = fit;
+        if (bpc >= 1) {
+            blocks = bpc * (h->cu_count > 0 ? h->cu_count : 256);
+            const int64_t cap = n_sent / kMinSentencesPerWave / w;
+            if (blocks > cap) blocks = (int)(cap < 1 ? 1 : cap);
+        }
+    }
     if (wpb <= 0) {  // the block size that keeps the most waves resident (smallest on ties)
         int best = 0;
         wpb = 1;
@@ -558,12 +572,14 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         default_shape(h, f.lds_per_wave, a->sent_end - a->sent_begin, wpb, blocks);
         if (!pers) wpb = a->waves_per_block > 0 ? a->waves_per_block : 1;
         if (wpb > kMaxWpb) return seterr(h, PAR_SGNS_EINVAL, "at most %d waves per block", kMaxWpb);
+        f.grab = a->sentences_per_dequeue > 0 ? a->sentences_per_dequeue : (h->tuned_grab > 0 ? h->tuned_grab : 1);
+        if (f.grab > 64) return seterr(h, PAR_SGNS_EINVAL, "sentences_per_dequeue must be 0..64");
         if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
         if (pers && !planned && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
         if (pers && a->use_graph) {
-            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0) | (f.rep_mfma ? 4 : 0), wpb, blocks, a->iter, 0,
-                       a->seed, a->train_words};
+            GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0) | (f.rep_mfma ? 4 : 0), wpb, blocks, a->iter,
+                       f.grab, 0, a->seed, a->train_words};
             std::memcpy(&k.alpha_bits, &a->alpha, 4);
             auto it = h->graphs.find(k);
             if (it == h->graphs.end()) {
@@ -622,6 +638,105 @@ int par_sgns_train(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_sta
 }
 
 int par_sgns_prepare(par_sgns_handle h, const par_sgns_train_args *a) { return train_impl(h, a, nullptr, true); }
+
+int par_sgns_autotune(par_sgns_handle h, const par_sgns_train_args *a, int32_t reps, par_sgns_tune_point *points,
+                      int32_t cap, int32_t *n_points, par_sgns_tune_point *best) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (!a || !best || !n_points || cap < 0 || (cap > 0 && !points)) return seterr(h, PAR_SGNS_EINVAL, "null argument");
+    if (a->mode != PAR_SGNS_PERSISTENT) return seterr(h, PAR_SGNS_EINVAL, "the autotuner times the persistent mode");
+    if (reps < 1 || reps > 64) return seterr(h, PAR_SGNS_EINVAL, "reps must be 1..64");
+    if (!h->have_exp || !h->have_weights) return seterr(h, PAR_SGNS_ESTATE, "exp table and weights must be set first");
+    HIPCHK(h, hipSetDevice(h->cfg.device));
+    int rc;
+    // the shape the first trial takes validates the arguments (and orders the range)
+    par_sgns_train_args t = *a;
+    t.use_graph = 0;
+    t.waves_per_block = 1;
+    t.blocks = h->cu_count > 0 ? h->cu_count : 256;
+    t.sentences_per_dequeue = 1;
+    if ((rc = train_impl(h, &t, nullptr, true))) return rc;
+    const bool small = (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) - 4096 &&
+                       h->cfg.batch_size <= 16 * sgns::kMfI;
+    const bool mfma = h->bf16 && small && a->repulsion != PAR_SGNS_REP_VECTOR;
+    const int lds = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld, mfma);
+    int max_wpcu = 0;
+    for (int k = 1; k <= kMaxWpb; k++) max_wpcu = std::max(max_wpcu, blocks_per_cu(lds, k) * k);
+    if (max_wpcu < 1) return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", lds);
+    std::vector<int> wpcus;
+    for (int w : {4, 6, 8})
+        if (w <= max_wpcu) wpcus.push_back(w);
+    if (std::find(wpcus.begin(), wpcus.end(), max_wpcu) == wpcus.end()) wpcus.push_back(max_wpcu);
+    // both tables aside: every trial starts from them and they come back at the end
+    const size_t bytes = (size_t)h->cfg.vocab_size * (size_t)h->ld * (h->bf16 ? 2 : 4);
+    char *s_in = nullptr, *s_out = nullptr;
+    if ((rc = dalloc(h, s_in, bytes))) return rc;
+    if ((rc = dalloc(h, s_out, bytes))) {
+        dfree(s_in);
+        return rc;
+    }
+    auto restore = [&]() -> hipError_t {
+        hipError_t e = hipMemcpyAsync(h->w_in, s_in, bytes, hipMemcpyDeviceToDevice, h->st);
+        if (e == hipSuccess) e = hipMemcpyAsync(h->w_out, s_out, bytes, hipMemcpyDeviceToDevice, h->st);
+        return e;
+    };
+    hipError_t e = hipMemcpyAsync(s_in, h->w_in, bytes, hipMemcpyDeviceToDevice, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s_out, h->w_out, bytes, hipMemcpyDeviceToDevice, h->st);
+    const int64_t n_sent = a->sent_end - a->sent_begin;
+    int32_t n = 0;
+    par_sgns_tune_point bp{};
+    bp.kernel_ms = -1.0;
+    for (int wpcu : wpcus) {
+        for (int wpb : {1, 2, 4, 8}) {
+            if (e != hipSuccess || rc) break;
+            if (wpcu % wpb || blocks_per_cu(lds, wpb) * wpb < wpcu) continue;
+            for (int grab : {1, 2, 4}) {
+                t.waves_per_block = wpb;
+                t.blocks = (wpcu / wpb) * (h->cu_count > 0 ? h->cu_count : 256);
+                const int64_t capb = n_sent / kMinSentencesPerWave / wpb;
+                if (t.blocks > capb) t.blocks = (int)(capb < 1 ? 1 : capb);
+                t.sentences_per_dequeue = grab;
+                std::vector<double> ms;
+                for (int r = 0; r < reps && e == hipSuccess && !rc; r++) {
+                    e = restore();
+                    if (e == hipSuccess && !(rc = train_impl(h, &t, nullptr, false))) ms.push_back(h->last_ms[0]);
+                }
+                if (e != hipSuccess || rc) break;
+                std::sort(ms.begin(), ms.end());
+                par_sgns_tune_point p{wpb, wpcu, grab, 0, ms[ms.size() / 2]};
+                if (n < cap) points[n] = p;
+                n++;
+                if (bp.kernel_ms < 0 || p.kernel_ms < bp.kernel_ms) bp = p;
+            }
+        }
+    }
+    if (e == hipSuccess) e = restore();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    dfree(s_in);
+    dfree(s_out);
+    if (rc) return rc;
+    if (e != hipSuccess) return seterr(h, PAR_SGNS_EDEVICE, "autotune: %s", hipGetErrorString(e));
+    h->tuned_wpb = bp.waves_per_block;
+    h->tuned_wpcu = bp.waves_per_cu;
+    h->tuned_grab = bp.sentences_per_dequeue;
+    *best = bp;
+    *n_points = n;
+    return 0;
+}
+
+int par_sgns_set_launch_shape(par_sgns_handle h, int32_t wpb, int32_t wpcu, int32_t grab) {
+    if (!h) return PAR_SGNS_EINVAL;
+    if (wpb == 0 && wpcu == 0 && grab == 0) {
+        h->tuned_wpb = h->tuned_wpcu = h->tuned_grab = 0;
+        return 0;
+    }
+    if (wpb < 1 || wpb > kMaxWpb || wpcu < 1 || wpcu > kMaxWavesCu || wpcu % wpb || grab < 1 || grab > 64)
+        return seterr(h, PAR_SGNS_EINVAL, "launch shape: waves per block 1..%d, waves per CU a multiple of it up to %d, "
+                      "sentences per dequeue 1..64", kMaxWpb, kMaxWavesCu);
+    h->tuned_wpb = wpb;
+    h->tuned_wpcu = wpcu;
+    h->tuned_grab = grab;
+    return 0;
+}
 
 static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg, int64_t nw,
                        int32_t nctx, int32_t nneg, int32_t flags, double *loss_out, int adjoint) {

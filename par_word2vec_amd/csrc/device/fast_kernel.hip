@@ -1016,17 +1016,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void s
     };
     const int64_t n = a.s_end - a.s_begin;
     if (a.persistent) {
+        // each dequeue takes `grab` consecutive queue slots (one atomic per
+        // grab sentences); every wave leaves once the queue is past n
+        const int grab = a.grab;
         for (;;) {
             unsigned long long qi = 0;
-            if (lane == 0) qi = atomicAdd(a.queue, 1ull);
-            const int64_t idx = readfirst_i64((int64_t)qi);
-            if (idx >= n) break;
-            const int64_t slot = a.s_begin + idx;
-            const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
-            const int64_t raw = a.plan_b ? 0 : (a.order ? a.queue_raw[slot] : a.sent_raw[s]);
-            SentCount c{0u, 0u, 0u, 0u, 0ull};
-            train_sentence<T, NP, NB, MF>(a, s, raw, lane, lds, loss_a, loss_r, c);
-            fold(c);
+            if (lane == 0) qi = atomicAdd(a.queue, (unsigned long long)grab);
+            const int64_t idx0 = readfirst_i64((int64_t)qi);
+            if (idx0 >= n) break;
+            const int64_t idx1 = idx0 + grab < n ? idx0 + grab : n;
+            for (int64_t idx = idx0; idx < idx1; idx++) {
+                const int64_t slot = a.s_begin + idx;
+                const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
+                const int64_t raw = a.plan_b ? 0 : (a.order ? a.queue_raw[slot] : a.sent_raw[s]);
+                SentCount c{0u, 0u, 0u, 0u, 0ull};
+                train_sentence<T, NP, NB, MF>(a, s, raw, lane, lds, loss_a, loss_r, c);
+                fold(c);
+            }
         }
     } else {
         const int64_t idx = (int64_t)blockIdx.x * (blockDim.x / kWave) + wave;

@@ -39,6 +39,7 @@ struct FastArgs {
     int32_t lds_per_wave;         // bytes
     int32_t lds_negs;             // byte offset of the batch's negatives in a wave's LDS (after the snapshot)
     int32_t persistent;
+    int32_t grab;                 // persistent: queue slots (sentences) a wave takes per dequeue, >= 1
     int32_t rounding;             // bf16 write-back: 0 stochastic, 1 nearest
     int32_t rep_mfma;             // bf16 tables: repulsion products on the matrix cores
     int32_t compute_loss;

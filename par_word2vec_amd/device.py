@@ -110,17 +110,37 @@ class SgnsDevice:
     def train(self, mode: str | int, sent_begin: int, sent_end: int, alpha: float, train_words: int,
               iters: int = 1, epoch: int = 0, seed: int = 1, compute_loss: bool = False,
               waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False, planned: bool = False,
-              prepare_only: bool = False, repulsion: str = "auto") -> dict:
+              prepare_only: bool = False, repulsion: str = "auto", sentences_per_dequeue: int = 0) -> dict:
         m = MODES[mode] if isinstance(mode, str) else int(mode)
         args = N.ParSgnsTrainArgs(m, iters, epoch, int(compute_loss), alpha, waves_per_block, blocks,
                                   int(use_graph), int(planned), REPULSIONS[repulsion], train_words, seed,
-                                  sent_begin, sent_end)
+                                  sent_begin, sent_end, sentences_per_dequeue, 0)
         if prepare_only:
             self._chk(self.lib.par_sgns_prepare(self._h, C.byref(args)), "prepare")
             return {}
         st = N.ParSgnsStats()
         self._chk(self.lib.par_sgns_train(self._h, C.byref(args), C.byref(st)), "train")
         return st.as_dict()
+
+    def autotune(self, sent_begin: int, sent_end: int, alpha: float, train_words: int, iters: int = 1,
+                 epoch: int = 0, seed: int = 1, reps: int = 3, repulsion: str = "auto") -> dict:
+        """Times the persistent kernel's launch shapes on the sentence range
+        (par_sgns_autotune; the tables are left as they were) and makes the
+        fastest the handle's default.  Returns {"best": point, "points": [...]}."""
+        args = N.ParSgnsTrainArgs(MODES["persistent"], iters, epoch, 0, alpha, 0, 0, 0, 0, REPULSIONS[repulsion],
+                                  train_words, seed, sent_begin, sent_end, 0, 0)
+        cap = 64
+        pts = (N.ParSgnsTunePoint * cap)()
+        n = C.c_int32(0)
+        best = N.ParSgnsTunePoint()
+        self._chk(self.lib.par_sgns_autotune(self._h, C.byref(args), reps, pts, cap, C.byref(n), C.byref(best)),
+                  "autotune")
+        return {"best": best.as_dict(), "points": [pts[i].as_dict() for i in range(min(n.value, cap))]}
+
+    def set_launch_shape(self, waves_per_block: int = 0, waves_per_cu: int = 0, sentences_per_dequeue: int = 0) -> None:
+        """Default persistent launch shape from a saved tuning (all 0: built-in default)."""
+        self._chk(self.lib.par_sgns_set_launch_shape(self._h, waves_per_block, waves_per_cu, sentences_per_dequeue),
+                  "set_launch_shape")
 
     # ------------------------------------------------- loss and its adjoint
     def _windows(self, fn, centre, ctx, neg, flags) -> float:
