@@ -31,6 +31,7 @@ struct Opts {
     float alpha = 0.025f, sample = 1e-3f;
     unsigned long long seed = 1;
     long long chunk_words = 10000000;
+    int autotune = 0;  // persistent mode: tune the launch shape on the first chunk, this many runs per shape
 };
 
 void usage() {
@@ -41,6 +42,7 @@ void usage() {
         "  -batch-size <int> (24) -binary <int> (0) -debug <int> (2) -save-vocab <file> -read-vocab <file>\n"
         "  -mode det|plain|persistent (persistent) -precision fp32|bf16 (fp32) -seed <int> -device <int>\n"
         "  -chunk-words <int> (10000000) -time-file <file> (PAR_Word2Vec_gpu_time)\n"
+        "  -autotune <runs> (0): persistent mode, tune the launch shape on the first chunk first\n"
         "  accepted and ignored: -threads -disk -gamma -max-num-sen\n");
 }
 
@@ -78,6 +80,7 @@ int main(int argc, char **argv) {
     str("-mode", o.mode);
     str("-precision", o.precision);
     num("-device", o.device);
+    num("-autotune", o.autotune);
     str("-time-file", o.time_file);
     if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = std::strtoull(argv[i + 1], nullptr, 10);
     if ((i = arg_pos("-chunk-words", argc, argv)) > 0 && i + 1 < argc) o.chunk_words = std::atoll(argv[i + 1]);
@@ -167,6 +170,20 @@ int main(int argc, char **argv) {
             if (par_sgns_set_corpus(h, tok.data(), nt, off.data(), raw.data(), S))
                 return fail("corpus", par_sgns_last_error(h));
             ta.epoch = ep;
+            if (ep == 0 && o.autotune > 0 && mode == PAR_SGNS_PERSISTENT && S > 0) {  // untimed
+                int64_t e = 1;
+                while (e < S && raw[e] - raw[0] < o.chunk_words) e++;
+                par_sgns_train_args tt = ta;
+                tt.sent_begin = 0;
+                tt.sent_end = e;
+                par_sgns_tune_point best{};
+                int32_t np = 0;
+                if (par_sgns_autotune(h, &tt, o.autotune, nullptr, 0, &np, &best))
+                    return fail("autotune", par_sgns_last_error(h));
+                std::printf("launch shape (%d shapes timed): %d waves per block, %d per CU, %d sentences per dequeue, "
+                            "%.3f ms on the first chunk\n", np, best.waves_per_block, best.waves_per_cu,
+                            best.sentences_per_dequeue, best.kernel_ms);
+            }
             auto t0 = std::chrono::steady_clock::now();
             int64_t b = 0;
             uint64_t tokens = 0;

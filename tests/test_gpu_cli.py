@@ -50,3 +50,17 @@ def test_cli_persistent_text_output(tmp_path):
     vals = np.array(lines[2].split()[1:], dtype=np.float64)
     assert vals.size == 32 and np.isfinite(vals).all()
     assert (tmp_path / "v.txt").read_text().startswith("</s> ")
+
+
+def test_cli_autotune(tmp_path):
+    """-autotune N times the launch shapes on the first chunk before training
+    (untimed) and reports the chosen one; the run then trains as usual."""
+    p = _corpus(tmp_path)
+    out = tmp_path / "vec.txt"
+    r = subprocess.run([str(CLI), "-train", p, "-output", str(out), "-size", "64", "-iter", "1", "-precision",
+                        "bf16", "-sample", "1e-4", "-autotune", "1", "-time-file", str(tmp_path / "t")],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "launch shape (" in r.stdout and "sentences per dequeue" in r.stdout
+    V, d = map(int, out.read_text().split("\n")[0].split())
+    assert d == 64 and V > 1
