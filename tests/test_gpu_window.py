@@ -175,3 +175,26 @@ def test_adjoint_vs_cpu_pair_update():
     np.testing.assert_allclose(-alpha * go[1], upd_out, rtol=1e-2, atol=1e-9)
     np.testing.assert_allclose(-alpha * gi[2], upd_in, rtol=1e-2, atol=1e-9)
     assert np.abs(upd_out).max() > 1e-6
+
+
+def test_gradient_error_per_batch(windows):
+    """Per-batch gradient error of each storage/arithmetic choice against the
+    FP64 gradient of the unrounded fp32 tables (one batch of 65,536 windows):
+    fp32 tables with fp32 sums, bf16 storage with fp32 sums, and bf16 storage
+    on the matrix cores.  Relative L2 error over both gradient tables; the bf16
+    rows carry the storage rounding of the tables (2^-9 relative per entry)."""
+    w_in, w_out, centre, ctx, neg = windows
+    gi_ref, go_ref = O.window_grad_f64(centre, ctx, neg, w_in, w_out)
+    ref = np.concatenate([gi_ref.ravel(), go_ref.ravel()])
+    errs = {}
+    for name, prec, flags in [("fp32", "fp32", 0), ("bf16", "bf16", 0), ("bf16-matrix", "bf16", WIN_MFMA)]:
+        with SgnsDevice(V, D, 5, NNEG, 24, prec) as dev:
+            dev.set_weights(w_in, w_out)  # bf16 handles round on upload
+            dev.loss_adjoint(centre, ctx, neg, flags=flags)
+            gi, go = dev.get_grads()
+        g = np.concatenate([gi.ravel(), go.ravel()]).astype(np.float64)
+        errs[name] = (np.linalg.norm(g - ref) / np.linalg.norm(ref), np.abs(g - ref).max() / np.abs(ref).max())
+    print("[grad error per batch] " + "; ".join(f"{k}: rel L2 {a:.2e}, max {b:.2e}" for k, (a, b) in errs.items()),
+          flush=True)
+    assert errs["fp32"][0] < 1e-5
+    assert errs["bf16"][0] < 1e-2 and errs["bf16-matrix"][0] < 1e-2
