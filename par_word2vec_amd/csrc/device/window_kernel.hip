@@ -149,9 +149,11 @@ hipError_t launch_t(const WindowArgs &a, int np, int adjoint, int fp64, hipStrea
 // zero rows up to 16), then
 //   F    = X Y^T            v_mfma_f32_16x16x32_bf16 (exact bf16 products, fp32 sums)
 //   G    = sigmoid(F) - label, split into bf16 hi + lo (about 16 significant bits)
-//   dY^T = X^T G            v_mfma_f32_16x16x16_bf16, twice (hi, lo)
-//   dX^T = Y^T G^T          idem, G^T through a 16 x 16 LDS tile
-// A 16x16 output tile holds, per lane, four consecutive dimensions of one row.
+//   dY   = G^T X            v_mfma_f32_16x16x16_bf16, twice (hi, lo)
+//   dX   = G Y              idem, G's A-operand layout through a 16 x 16 LDS tile
+// With the gradient as the A operand an output tile holds, per lane, one
+// dimension of four rows, so each float atomic instruction adds four rows x
+// 16 consecutive floats (four 64-byte requests).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef short s16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -243,8 +245,13 @@ __global__ __launch_bounds__(256) void sgns_window_mfma_kernel(WindowArgs a, int
             gl[mt] = s16x4_t{(short)lb[0], (short)lb[1], (short)lb[2], (short)lb[3]};
         }
         if constexpr (ADJ) {
-            // ---- dY^T = X^T G: lane (r16 = y, g4) gets dims 4*g4 + r of row y
-            const int32_t yrow = r16 < ny ? (r16 == 0 ? a.centre[w] : a.neg[w * a.nneg + (r16 - 1)]) : -1;
+            // ---- dY = G^T X: lane (r16 = dimension, g4) gets rows y = 4*g4 + r
+            int32_t yrow[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int y = 4 * g4 + r;
+                yrow[r] = y < ny ? (y == 0 ? a.centre[w] : a.neg[w * a.nneg + (y - 1)]) : -1;
+            }
             for (int jt = 0; jt < JT; jt++) {
                 f32x4_t dy = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -252,18 +259,15 @@ __global__ __launch_bounds__(256) void sgns_window_mfma_kernel(WindowArgs a, int
                     s16x4_t xv;
 #pragma unroll
                     for (int r = 0; r < 4; r++) xv[r] = (short)xs[(mt * 16 + 4 * g4 + r) * SC + jt * 16 + r16];
-                    dy = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xv, gh[mt], dy, 0, 0, 0);
-                    dy = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xv, gl[mt], dy, 0, 0, 0);
+                    dy = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gh[mt], xv, dy, 0, 0, 0);
+                    dy = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gl[mt], xv, dy, 0, 0, 0);
                 }
-                const int j = jt * 16 + 4 * g4;
-                if (yrow >= 0) {
-                    float *p = a.g_out + (int64_t)yrow * ld + j;
+                const int j = jt * 16 + r16;
 #pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (j + r < d && dy[r] != 0.f) unsafeAtomicAdd(p + r, dy[r]);
-                }
+                for (int r = 0; r < 4; r++)
+                    if (yrow[r] >= 0 && j < d && dy[r] != 0.f) unsafeAtomicAdd(a.g_out + (int64_t)yrow[r] * ld + j, dy[r]);
             }
-            // ---- dX^T = Y^T G^T: lane (r16 = context, g4) gets dims 4*g4 + r
+            // ---- dX = G Y: lane (r16 = dimension, g4) gets contexts mt*16 + 4*g4 + r
 #pragma unroll
             for (int mt = 0; mt < MT; mt++) {
                 // G^T tile of this context block (hi and lo) through LDS
@@ -275,21 +279,22 @@ __global__ __launch_bounds__(256) void sgns_window_mfma_kernel(WindowArgs a, int
                 const s16x4_t th = __builtin_bit_cast(s16x4_t, *reinterpret_cast<const uint2 *>(tr + r16 * kWinTS + 4 * g4));
                 const s16x4_t tl =
                     __builtin_bit_cast(s16x4_t, *reinterpret_cast<const uint2 *>(tr + 16 * kWinTS + r16 * kWinTS + 4 * g4));
-                const int i = mt * 16 + r16;
-                const int32_t xrow = i < nctx ? a.ctx[w * nctx + i] : -1;
+                int32_t xrow[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = mt * 16 + 4 * g4 + r;
+                    xrow[r] = i < nctx ? a.ctx[w * nctx + i] : -1;
+                }
                 for (int jt = 0; jt < JT; jt++) {
                     s16x4_t yv;
 #pragma unroll
                     for (int r = 0; r < 4; r++) yv[r] = (short)ys[(4 * g4 + r) * SC + jt * 16 + r16];
-                    f32x4_t dx = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(yv, th, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-                    dx = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(yv, tl, dx, 0, 0, 0);
-                    const int j = jt * 16 + 4 * g4;
-                    if (xrow >= 0) {
-                        float *p = a.g_in + (int64_t)xrow * ld + j;
+                    f32x4_t dx = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(th, yv, f32x4_t{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    dx = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(tl, yv, dx, 0, 0, 0);
+                    const int j = jt * 16 + r16;
 #pragma unroll
-                        for (int r = 0; r < 4; r++)
-                            if (j + r < d && dx[r] != 0.f) unsafeAtomicAdd(p + r, dx[r]);
-                    }
+                    for (int r = 0; r < 4; r++)
+                        if (xrow[r] >= 0 && j < d && dx[r] != 0.f) unsafeAtomicAdd(a.g_in + (int64_t)xrow[r] * ld + j, dx[r]);
                 }
             }
         }
