@@ -44,6 +44,7 @@ class Options:
     chunk_words: int = 10_000_000  # raw words per training launch (hipGraph chunk)
     use_graph: bool = True
     compute_loss: bool = False
+    autotune: int = 0             # persistent mode: tune the launch shape on the first chunk (runs per shape)
 
 
 def chunk_bounds(sent_raw: np.ndarray, chunk_words: int) -> list[tuple[int, int]]:
@@ -106,6 +107,12 @@ class ParWord2Vec:
         o = self.opts
         for ep in range(o.iter):
             chunks = self.prepare_epoch(ep)
+            if ep == 0 and o.autotune > 0 and o.mode == "persistent" and chunks:  # untimed
+                b, e = chunks[0]
+                self.tuning = self.dev.autotune(b, e, o.alpha, self.train_words, iters=o.iter, seed=o.seed,
+                                                reps=o.autotune)
+                if o.debug > 0:
+                    print(f"launch shape: {self.tuning['best']}", flush=True)
             t0 = time.perf_counter()
             agg: dict = {}
             for b, e in chunks:
