@@ -45,6 +45,8 @@ def zipf1m():
     (64, 3, 2, 7, 2, 1e-4),
     (130, 8, 5, 16, 1, 0.0),
     (300, 5, 10, 24, 1, 1e-4),
+    (512, 3, 5, 24, 1, 1e-3),   # the largest dimension
+    (64, 31, 3, 64, 1, 1e-3),   # the largest window and batch
 ])
 def test_det_bitwise_vs_oracle(dim, window, negative, batch, iters, sample):
     print(f"[det] dim={dim} W={window} neg={negative} B={batch} iter={iters} sample={sample}", flush=True)
@@ -93,6 +95,9 @@ def test_det_zipf1m_full(zipf1m):
     ((100, "fp32"), 5, 0, 24, 1e-3),
     ((75, "fp32"), 4, 6, 20, 1e-3),    # odd dimension: a pair straddles the row's end
     ((200, "fp32"), 5, 5, 24, 1e-3),   # two pairs per lane, lanes 36..63 past the row
+    ((512, "fp32"), 3, 5, 24, 1e-3),   # the largest dimension: four pairs per lane
+    ((64, "fp32"), 31, 3, 64, 1e-3),   # the largest window (63 ring slots) and batch
+    ((48, "fp32"), 6, 64, 8, 1e-3),    # the most negatives: up to 704 per batch
 ])
 def test_hogwild_kernel_exact_schedule(precision_dim, window, negative, batch, sample):
     """The Hogwild kernel, one wavefront, driven by the oracle's schedule:
@@ -120,8 +125,12 @@ def test_hogwild_kernel_exact_schedule(precision_dim, window, negative, batch, s
     assert np.abs(fo - do).max() <= 1e-4 * np.abs(do - w_out0).max()
     for k in ("pairs", "negatives", "rep_pairs", "tokens"):
         assert fs[k] == ds[k]
-    np.testing.assert_allclose(fs["loss_attr"], ds["loss_attr"], rtol=1e-6)
-    np.testing.assert_allclose(fs["loss_rep"], ds["loss_rep"], rtol=1e-6)
+    # the float-reordering error of a repulsion sum grows like the square root
+    # of its length, K <= (2W-1)*negative terms per batch row: 1e-6 up to 90
+    # terms (every workload of the reference's flags), scaled beyond
+    rtol = 1e-6 * max(1.0, ((2 * window - 1) * negative / 90) ** 0.5)
+    np.testing.assert_allclose(fs["loss_attr"], ds["loss_attr"], rtol=rtol)
+    np.testing.assert_allclose(fs["loss_rep"], ds["loss_rep"], rtol=rtol)
 
 
 def _planned_runs(vocab, stream, dim, window, negative, batch, sample, runs):
@@ -157,6 +166,7 @@ def _rms(x):
     (64, 8, 25, 24, False),   # up to 375 negatives: several negative groups
     (300, 5, 10, 24, True),   # one-word sentences: repulsion only
     (75, 4, 6, 20, False),    # odd dimension: the last written pair straddles d
+    (512, 3, 5, 32, False),   # the largest dimension on the matrix path
 ])
 def test_matrix_repulsion_exact_schedule(dim, window, negative, batch, single):
     """bf16 tables, round-to-nearest write-backs, the oracle's exact schedule
