@@ -583,6 +583,11 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
         if (lane == (c & pmask)) pos_slot = slot;
     };
 
+    // The attraction is a chain of dependent pairs (latency bound); the
+    // repulsion has independent matrix, LDS and atomic work.  The attraction
+    // runs at wave priority 1, so when both waves of a SIMD can issue, its
+    // chain goes first and the other wave's repulsion fills the gaps.
+    __builtin_amdgcn_s_setprio(1);
     // ---- attraction
     // The W_in rows of window [p-W, p+W] live in LDS slots keyed by word: a
     // word occurring twice in the window shares one slot, so every pair sees
@@ -812,6 +817,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
         }
     }
 
+    __builtin_amdgcn_s_setprio(0);
     // ---- repulsion (cpu:571-679)
     // Per batch of B positions: K shared negatives; logits G = N C^T against
     // the batch rows C, label-0 gradient, then W_out[n_k] += sum_i G_ki c_i and
