@@ -27,6 +27,27 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
+// Adds a row held as pairs (lane l: pairs l, l+64, ...) to a dense fp32
+// gradient row with one float per lane and atomic: lane l takes float l of a
+// 128-float block from lane l/2 and float 64 + l from lane 32 + l/2 (shuffles
+// through ds_bpermute), so each atomic instruction adds 64 consecutive floats
+// -- four 64-byte requests, where a pair per lane (two atomics 8 bytes apart)
+// costs eight per instruction.  Pairs past the row hold zeros.
+template <int NP>
+__device__ __forceinline__ void add_row_f32(float *row, const float2 (&v)[NP], int d, int lane) {
+    const int s0 = lane >> 1, s1 = 32 + (lane >> 1);
+    const bool odd = lane & 1;
+#pragma unroll
+    for (int t = 0; t < NP; t++) {
+        const float a0 = __shfl(v[t].x, s0), a1 = __shfl(v[t].y, s0);
+        const float b0 = __shfl(v[t].x, s1), b1 = __shfl(v[t].y, s1);
+        const float va = odd ? a1 : a0, vb = odd ? b1 : b0;
+        const int f = 128 * t + lane;
+        if (f < d && va != 0.f) unsafeAtomicAdd(row + f, va);
+        if (f + 64 < d && vb != 0.f) unsafeAtomicAdd(row + f + 64, vb);
+    }
+}
+
 template <typename T, int NP, typename ACC, bool ADJ>
 __global__ __launch_bounds__(256) void sgns_window_kernel(WindowArgs a) {
     using PT = typename Pair<T>::P;
@@ -84,31 +105,13 @@ __global__ __launch_bounds__(256) void sgns_window_kernel(WindowArgs a) {
                     }
                 }
             }
-            if (ADJ) {
-#pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    if (q < nq) {
-                        float *p = a.g_in + (int64_t)xrow * ld + 2 * q;
-                        if (dx[t].x != 0.f) unsafeAtomicAdd(p, dx[t].x);
-                        if (dx[t].y != 0.f) unsafeAtomicAdd(p + 1, dx[t].y);
-                    }
-                }
-            }
+            if (ADJ) add_row_f32<NP>(a.g_in + (int64_t)xrow * ld, dx, a.d, lane);
         }
         if (ADJ) {
 #pragma unroll
             for (int k = 0; k < kMaxY; k++) {
                 if (k >= ny) continue;
-#pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    if (q < nq) {
-                        float *p = a.g_out + (int64_t)yrow[k] * ld + 2 * q;
-                        if (dy[k][t].x != 0.f) unsafeAtomicAdd(p, dy[k][t].x);
-                        if (dy[k][t].y != 0.f) unsafeAtomicAdd(p + 1, dy[k][t].y);
-                    }
-                }
+                add_row_f32<NP>(a.g_out + (int64_t)yrow[k] * ld, dy[k], a.d, lane);
             }
         }
     }
