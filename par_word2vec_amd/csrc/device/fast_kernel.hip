@@ -46,12 +46,6 @@ constexpr int kUnusedSlot = -(1 << 30);
 // kept, never overwritten.  A step that rounds to zero issues no write.
 typedef __bf16 __attribute__((ext_vector_type(2))) bf16x2_t;
 
-__device__ __forceinline__ void wb_pair(float *w, int64_t row, int ld, int q, float2 base, float2 dlt, uint32_t &,
-                                        int) {
-    float *p = w + row * ld + 2 * q;
-    if (dlt.x != 0.0f) unsafeAtomicAdd(p, dlt.x);
-    if (dlt.y != 0.0f) unsafeAtomicAdd(p + 1, dlt.y);
-}
 // The packed bf16 step s - base from base to s = cur rounded to bf16 (0 if
 // the pair does not move).
 __device__ __forceinline__ uint32_t bf16_step_to(float2 base, float2 cur, uint32_t &rs, int nearest) {
@@ -104,10 +98,30 @@ __device__ __forceinline__ void to_f(const typename Pair<T>::P (&in)[NP], float2
 template <typename T, int NP>
 __device__ __forceinline__ void store_delta(T *w, int64_t row, int ld, int nq, int lane, const float2 (&base)[NP],
                                             const float2 (&dlt)[NP], uint32_t &rs, int nearest) {
+    if constexpr (sizeof(T) == 4) {
+        // fp32: one float per lane and atomic.  Lane l takes float l of a
+        // 128-float block from lane l/2 and float 64 + l from lane 32 + l/2
+        // (shuffles: every lane of the wave takes part), so an instruction adds
+        // 64 consecutive floats -- four 64-byte requests, where a pair per lane
+        // (two atomics 8 bytes apart) costs eight.  Pairs past the row are zero.
+        float *p = reinterpret_cast<float *>(w) + row * ld;
+        const int s0 = lane >> 1, s1 = 32 + (lane >> 1), n = 2 * nq;
+        const bool odd = lane & 1;
 #pragma unroll
-    for (int t = 0; t < NP; t++) {
-        const int q = lane + kWave * t;
-        if (t < NP - 1 || q < nq) wb_pair(w, row, ld, q, base[t], dlt[t], rs, nearest);
+        for (int t = 0; t < NP; t++) {
+            const float a0 = __shfl(dlt[t].x, s0), a1 = __shfl(dlt[t].y, s0);
+            const float b0 = __shfl(dlt[t].x, s1), b1 = __shfl(dlt[t].y, s1);
+            const float va = odd ? a1 : a0, vb = odd ? b1 : b0;
+            const int f = 128 * t + lane;
+            if (f < n && va != 0.0f) unsafeAtomicAdd(p + f, va);
+            if (f + 64 < n && vb != 0.0f) unsafeAtomicAdd(p + f + 64, vb);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            const int q = lane + kWave * t;
+            if (t < NP - 1 || q < nq) wb_pair(w, row, ld, q, base[t], dlt[t], rs, nearest);
+        }
     }
 }
 
@@ -562,12 +576,10 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
         if constexpr (MF) {
             store_value_rsrc<NP>(rin_w, (uint32_t)row, row_bytes, nq, lane, u0, u, rs, a.rounding);
         } else {
+            float2 du[NP];
 #pragma unroll
-            for (int t = 0; t < NP; t++) {
-                const int q = lane + kWave * t;
-                if (t < NP - 1 || q < nq)
-                    wb_pair(w_in, row, ld, q, u0[t], make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y), rs, a.rounding);
-            }
+            for (int t = 0; t < NP; t++) du[t] = make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y);
+            store_delta<T, NP>(w_in, row, ld, nq, lane, u0, du, rs, a.rounding);
         }
     };
     auto flush = [&](int slot) {
