@@ -464,6 +464,16 @@ static int blocks_per_cu(int lds_per_wave, int wpb) {
     return b;
 }
 
+// Whether the matrix-core repulsion can run: it addresses W_out with 32-bit
+// byte offsets and holds a batch's positions in one group of 32.
+static bool mfma_fits(par_sgns_handle h) {
+    return (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) - 4096 &&
+           h->cfg.batch_size <= 16 * sgns::kMfI;
+}
+static bool use_mfma(par_sgns_handle h, int32_t repulsion) {
+    return h->bf16 && mfma_fits(h) && repulsion != PAR_SGNS_REP_VECTOR;
+}
+
 static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, int &wpb, int &blocks) {
     if (wpb <= 0 && h->tuned_wpb > 0) wpb = h->tuned_wpb;
     if (blocks <= 0 && h->tuned_wpcu > 0) {  // the tuned resident waves per CU, where they fit
@@ -556,13 +566,9 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         f.B = h->cfg.batch_size; f.V = h->cfg.vocab_size; f.kmax = h->kmax;
         if (a->repulsion < PAR_SGNS_REP_AUTO || a->repulsion > PAR_SGNS_REP_MATRIX)
             return seterr(h, PAR_SGNS_EINVAL, "repulsion must be AUTO, VECTOR or MATRIX");
-        // the matrix-core path addresses W_out with 32-bit byte offsets and
-        // holds a batch's positions in one group of 32
-        const bool small = (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) - 4096 &&
-                           h->cfg.batch_size <= 16 * sgns::kMfI;
-        if (a->repulsion == PAR_SGNS_REP_MATRIX && (!h->bf16 || !small))
+        if (a->repulsion == PAR_SGNS_REP_MATRIX && (!h->bf16 || !mfma_fits(h)))
             return seterr(h, PAR_SGNS_EINVAL, "matrix-core repulsion runs on bf16 tables below 4 GiB with batch size <= 32");
-        f.rep_mfma = h->bf16 && small && a->repulsion != PAR_SGNS_REP_VECTOR;
+        f.rep_mfma = use_mfma(h, a->repulsion);
         f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld, f.rep_mfma);
         f.lds_negs = sgns::fast_lds_negs(h->nq, h->bf16, h->cfg.batch_size, h->ld, f.rep_mfma);
         f.persistent = pers ? 1 : 0; f.rounding = h->cfg.rounding; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
@@ -655,10 +661,8 @@ int par_sgns_autotune(par_sgns_handle h, const par_sgns_train_args *a, int32_t r
     t.blocks = h->cu_count > 0 ? h->cu_count : 256;
     t.sentences_per_dequeue = 1;
     if ((rc = train_impl(h, &t, nullptr, true))) return rc;
-    const bool small = (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u < (1ull << 32) - 4096 &&
-                       h->cfg.batch_size <= 16 * sgns::kMfI;
-    const bool mfma = h->bf16 && small && a->repulsion != PAR_SGNS_REP_VECTOR;
-    const int lds = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld, mfma);
+    const int lds = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld,
+                                            use_mfma(h, a->repulsion));
     int max_wpcu = 0;
     for (int k = 1; k <= kMaxWpb; k++) max_wpcu = std::max(max_wpcu, blocks_per_cu(lds, k) * k);
     if (max_wpcu < 1) return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", lds);
