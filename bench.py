@@ -11,7 +11,9 @@ Both variants train the same epoch from the same InitNet weights:
   fp32 plain      one wavefront per sentence, fp32 tables, float atomics
   bf16 persistent work-queue kernel, bf16 tables, fp32 math, stochastic
                   rounding, repulsion products on the matrix cores, hipGraph
-                  per chunk
+                  per chunk, launched with the saved launch-shape tuning of the
+                  workload (par_word2vec_amd/tuned/; --autotune tunes here,
+                  before the warmup)
 `value` is the bf16 persistent rate in Mwords/s: raw corpus words (every
 token incl. those removed by subsampling and "</s>") per second of the timed
 steps.  The planted-cluster recall@10 of both runs is reported beside it.
