@@ -436,13 +436,19 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                     }
                     f32x4_t nX = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, nY = nX;
                     if (ut < UT) {  // uniform
+                        // dN's products first: this tile writes them back next,
+                        // and the dC products issued after cover their latency
+#pragma unroll
+                        for (int it = 0; it < kMfI; it++) {
+                            if (it >= IT) continue;
+                            nX = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvX[it], nX, 0, 0, 0);
+                            nY = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvY[it], nY, 0, 0, 0);
+                        }
 #pragma unroll
                         for (int it = 0; it < kMfI; it++) {
                             if (it >= IT) continue;
                             cX[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], bX, cX[it], 0, 0, 0);
                             cY[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], bY, cY[it], 0, 0, 0);
-                            nX = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvX[it], nX, 0, 0, 0);
-                            nY = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gt[ut][it], cvY[it], nY, 0, 0, 0);
                         }
                     }
                     write_pair(rout, ut < UT, offN[ut], base, nX, nY, dimp, odd, rs, a.rounding);
