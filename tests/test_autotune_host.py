@@ -33,3 +33,15 @@ def test_apply_passes_the_best_shape():
 
     b = A.apply(Dev(), {"best": {"waves_per_block": 4, "waves_per_cu": 8, "sentences_per_dequeue": 1, "kernel_ms": 2.0}})
     assert calls == [(4, 8, 1)] and b["waves_per_cu"] == 8
+
+
+def test_saved_tunings_match_their_workloads():
+    """The committed tunings apply to the workloads they were measured on
+    (bench.py then launches with them instead of the built-in default)."""
+    from par_word2vec_amd.corpora import WORKLOADS
+    for name in ("1b", "text8", "sbm"):
+        t = A.load(name, WORKLOADS[name], "bf16")
+        assert t is not None, name
+        b = t["best"]
+        assert 1 <= b["waves_per_block"] <= 8 and b["waves_per_cu"] % b["waves_per_block"] == 0
+        assert b in t["points"]
