@@ -37,6 +37,16 @@ def test_create_rejects_bad_config_without_gpu():
     assert lib.par_sgns_create(C.byref(cfg), C.byref(h)) == -1 and not h.value
 
 
+def test_launch_shape_calls_reject_a_null_handle_without_gpu():
+    import ctypes as C
+    lib = N.sgns()
+    args = N.ParSgnsTrainArgs()
+    best = N.ParSgnsTunePoint()
+    n = C.c_int32(0)
+    assert lib.par_sgns_autotune(None, C.byref(args), 1, None, 0, C.byref(n), C.byref(best)) == -1
+    assert lib.par_sgns_set_launch_shape(None, 1, 8, 1) == -1
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(N, "LIB_DIR", tmp_path)
     with pytest.raises(N.NativeLibraryMissing):
