@@ -53,15 +53,16 @@ def test_sentences_per_dequeue_counts(prec):
             dev.train("persistent", 0, n_sent, 0.025, T, sentences_per_dequeue=65)
 
 
-def test_autotune_restores_tables_and_picks_fastest():
+@pytest.mark.parametrize("prec", ["bf16", "fp32"])
+def test_autotune_restores_tables_and_picks_fastest(prec):
     vocab, counts, T, csr = _corpus(seed=5)
     n_sent = csr.sent_off.size - 1
-    with _device(vocab, counts, csr, "bf16") as dev:
+    with _device(vocab, counts, csr, prec) as dev:
         w0_in, w0_out = dev.get_weights()
         dev.train("persistent", 0, n_sent, 0.025, T, seed=3)  # move the tables off their initial values
         w1_in, w1_out = dev.get_weights()
         assert not np.array_equal(w1_out, w0_out)
-        print("[autotune] bf16 dim 300", flush=True)
+        print(f"[autotune] {prec} dim 300", flush=True)
         res = dev.autotune(0, n_sent, 0.025, T, seed=3, reps=1)
         w2_in, w2_out = dev.get_weights()
         assert np.array_equal(w2_in, w1_in) and np.array_equal(w2_out, w1_out)
