@@ -84,7 +84,9 @@ typedef struct par_sgns_train_args {
     int64_t sent_end;     /* (deterministic mode: of the uploaded plan) */
     int32_t sentences_per_dequeue; /* persistent mode: queue slots a wavefront takes per
                                       dequeue, 1..64; 0 = tuned default (1 untuned) */
-    int32_t reserved;
+    int32_t min_sentences_per_wave; /* persistent mode, default grid: at least this many
+                                       sentences per wavefront (fewer waves on small ranges:
+                                       Hogwild staleness); 0 = 16 */
 } par_sgns_train_args;
 
 typedef struct par_sgns_stats {

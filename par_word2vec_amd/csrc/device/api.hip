@@ -474,7 +474,9 @@ static bool use_mfma(par_sgns_handle h, int32_t repulsion) {
     return h->bf16 && mfma_fits(h) && repulsion != PAR_SGNS_REP_VECTOR;
 }
 
-static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, int &wpb, int &blocks) {
+// floor: sentences per wave (the caller's, or kMinSentencesPerWave)
+static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, int64_t floor, int &wpb,
+                          int &blocks) {
     if (wpb <= 0 && h->tuned_wpb > 0) wpb = h->tuned_wpb;
     if (blocks <= 0 && h->tuned_wpcu > 0) {  // the tuned resident waves per CU, where they fit
         const int w = wpb > 0 ? wpb : 1;
@@ -483,7 +485,7 @@ static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, i
         if (bpc > fit) bpc = fit;
         if (bpc >= 1) {
             blocks = bpc * (h->cu_count > 0 ? h->cu_count : 256);
-            const int64_t cap = n_sent / kMinSentencesPerWave / w;
+            const int64_t cap = n_sent / floor / w;
             if (blocks > cap) blocks = (int)(cap < 1 ? 1 : cap);
         }
     }
@@ -502,7 +504,7 @@ static void default_shape(par_sgns_handle h, int lds_per_wave, int64_t n_sent, i
         int bpc = blocks_per_cu(lds_per_wave, wpb);
         if (bpc < 1) bpc = 1;
         blocks = bpc * (h->cu_count > 0 ? h->cu_count : 256);
-        const int64_t cap = n_sent / kMinSentencesPerWave / wpb;
+        const int64_t cap = n_sent / floor / wpb;
         if (blocks > cap) blocks = (int)(cap < 1 ? 1 : cap);
     }
 }
@@ -575,7 +577,9 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         f.a0 = a->alpha; f.T = a->train_words; f.seed = a->seed; f.s_begin = a->sent_begin; f.s_end = a->sent_end;
         f.queue = h->queue; f.loss = h->loss; f.counters = h->counters;
         int wpb = pers ? a->waves_per_block : 1, blocks = a->blocks;
-        default_shape(h, f.lds_per_wave, a->sent_end - a->sent_begin, wpb, blocks);
+        if (a->min_sentences_per_wave < 0) return seterr(h, PAR_SGNS_EINVAL, "min_sentences_per_wave must be >= 0");
+        const int64_t floor = a->min_sentences_per_wave > 0 ? a->min_sentences_per_wave : kMinSentencesPerWave;
+        default_shape(h, f.lds_per_wave, a->sent_end - a->sent_begin, floor, wpb, blocks);
         if (!pers) wpb = a->waves_per_block > 0 ? a->waves_per_block : 1;
         if (wpb > kMaxWpb) return seterr(h, PAR_SGNS_EINVAL, "at most %d waves per block", kMaxWpb);
         f.grab = a->sentences_per_dequeue > 0 ? a->sentences_per_dequeue : (h->tuned_grab > 0 ? h->tuned_grab : 1);
@@ -696,7 +700,7 @@ int par_sgns_autotune(par_sgns_handle h, const par_sgns_train_args *a, int32_t r
             for (int grab : {1, 2, 4}) {
                 t.waves_per_block = wpb;
                 t.blocks = (wpcu / wpb) * (h->cu_count > 0 ? h->cu_count : 256);
-                const int64_t capb = n_sent / kMinSentencesPerWave / wpb;
+                const int64_t capb = n_sent / (a->min_sentences_per_wave > 0 ? a->min_sentences_per_wave : kMinSentencesPerWave) / wpb;
                 if (t.blocks > capb) t.blocks = (int)(capb < 1 ? 1 : capb);
                 t.sentences_per_dequeue = grab;
                 std::vector<double> ms;

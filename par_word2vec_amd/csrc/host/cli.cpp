@@ -32,6 +32,7 @@ struct Opts {
     unsigned long long seed = 1;
     long long chunk_words = 10000000;
     int autotune = 0;  // persistent mode: tune the launch shape on the first chunk, this many runs per shape
+    int min_sent_wave = 0;  // persistent mode: staleness floor of the default grid (0 = 16)
 };
 
 void usage() {
@@ -43,6 +44,7 @@ void usage() {
         "  -mode det|plain|persistent (persistent) -precision fp32|bf16 (fp32) -seed <int> -device <int>\n"
         "  -chunk-words <int> (10000000) -time-file <file> (PAR_Word2Vec_gpu_time)\n"
         "  -autotune <runs> (0): persistent mode, tune the launch shape on the first chunk first\n"
+        "  -min-sentences-per-wave <int> (16): persistent mode, fewer waves on small corpora\n"
         "  accepted and ignored: -threads -disk -gamma -max-num-sen\n");
 }
 
@@ -81,6 +83,7 @@ int main(int argc, char **argv) {
     str("-precision", o.precision);
     num("-device", o.device);
     num("-autotune", o.autotune);
+    num("-min-sentences-per-wave", o.min_sent_wave);
     str("-time-file", o.time_file);
     if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = std::strtoull(argv[i + 1], nullptr, 10);
     if ((i = arg_pos("-chunk-words", argc, argv)) > 0 && i + 1 < argc) o.chunk_words = std::atoll(argv[i + 1]);
@@ -136,6 +139,7 @@ int main(int argc, char **argv) {
     ta.train_words = T;
     ta.seed = o.seed;
     ta.use_graph = mode == PAR_SGNS_PERSISTENT;
+    ta.min_sentences_per_wave = o.min_sent_wave;
     par_sgns_stats st{};
     if (mode == PAR_SGNS_DETERMINISTIC) {
         ph_plan *pl = ph_plan_build(stream.data(), n, counts.data(), V, T, o.window, o.negative, o.batch, o.iter,

@@ -110,11 +110,12 @@ class SgnsDevice:
     def train(self, mode: str | int, sent_begin: int, sent_end: int, alpha: float, train_words: int,
               iters: int = 1, epoch: int = 0, seed: int = 1, compute_loss: bool = False,
               waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False, planned: bool = False,
-              prepare_only: bool = False, repulsion: str = "auto", sentences_per_dequeue: int = 0) -> dict:
+              prepare_only: bool = False, repulsion: str = "auto", sentences_per_dequeue: int = 0,
+              min_sentences_per_wave: int = 0) -> dict:
         m = MODES[mode] if isinstance(mode, str) else int(mode)
         args = N.ParSgnsTrainArgs(m, iters, epoch, int(compute_loss), alpha, waves_per_block, blocks,
                                   int(use_graph), int(planned), REPULSIONS[repulsion], train_words, seed,
-                                  sent_begin, sent_end, sentences_per_dequeue, 0)
+                                  sent_begin, sent_end, sentences_per_dequeue, min_sentences_per_wave)
         if prepare_only:
             self._chk(self.lib.par_sgns_prepare(self._h, C.byref(args)), "prepare")
             return {}
@@ -123,12 +124,13 @@ class SgnsDevice:
         return st.as_dict()
 
     def autotune(self, sent_begin: int, sent_end: int, alpha: float, train_words: int, iters: int = 1,
-                 epoch: int = 0, seed: int = 1, reps: int = 3, repulsion: str = "auto") -> dict:
+                 epoch: int = 0, seed: int = 1, reps: int = 3, repulsion: str = "auto",
+                 min_sentences_per_wave: int = 0) -> dict:
         """Times the persistent kernel's launch shapes on the sentence range
         (par_sgns_autotune; the tables are left as they were) and makes the
         fastest the handle's default.  Returns {"best": point, "points": [...]}."""
         args = N.ParSgnsTrainArgs(MODES["persistent"], iters, epoch, 0, alpha, 0, 0, 0, 0, REPULSIONS[repulsion],
-                                  train_words, seed, sent_begin, sent_end, 0, 0)
+                                  train_words, seed, sent_begin, sent_end, 0, min_sentences_per_wave)
         cap = 64
         pts = (N.ParSgnsTunePoint * cap)()
         n = C.c_int32(0)

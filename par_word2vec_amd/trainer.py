@@ -45,6 +45,7 @@ class Options:
     use_graph: bool = True
     compute_loss: bool = False
     autotune: int = 0             # persistent mode: tune the launch shape on the first chunk (runs per shape)
+    min_sentences_per_wave: int = 0  # persistent mode: staleness floor of the default grid (0 = 16)
 
 
 def chunk_bounds(sent_raw: np.ndarray, chunk_words: int) -> list[tuple[int, int]]:
@@ -101,7 +102,8 @@ class ParWord2Vec:
     def run_chunk(self, epoch: int, b: int, e: int) -> dict:
         o = self.opts
         return self.dev.train(o.mode, b, e, o.alpha, self.train_words, iters=o.iter, epoch=epoch, seed=o.seed,
-                              compute_loss=o.compute_loss, use_graph=o.use_graph and o.mode == "persistent")
+                              compute_loss=o.compute_loss, use_graph=o.use_graph and o.mode == "persistent",
+                              min_sentences_per_wave=o.min_sentences_per_wave)
 
     def train(self) -> list[dict]:
         o = self.opts
@@ -110,7 +112,7 @@ class ParWord2Vec:
             if ep == 0 and o.autotune > 0 and o.mode == "persistent" and chunks:  # untimed
                 b, e = chunks[0]
                 self.tuning = self.dev.autotune(b, e, o.alpha, self.train_words, iters=o.iter, seed=o.seed,
-                                                reps=o.autotune)
+                                                reps=o.autotune, min_sentences_per_wave=o.min_sentences_per_wave)
                 if o.debug > 0:
                     print(f"launch shape: {self.tuning['best']}", flush=True)
             t0 = time.perf_counter()

@@ -17,7 +17,7 @@ import pytest
 
 import oracle as O
 from par_word2vec_amd import host as H
-from par_word2vec_amd.device import SgnsDevice
+from par_word2vec_amd.device import DeviceError, SgnsDevice
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +125,22 @@ def test_empty_ranges_train_nothing(ragged):
                 assert st["loss_attr"] == 0.0 and st["loss_rep"] == 0.0
             after = dev.get_weights()
         assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+
+
+@pytest.mark.parametrize("floor", [1, 4, 100000])
+def test_sentences_per_wave_floor(ragged, floor):
+    """min_sentences_per_wave only shapes the default persistent grid: every
+    kept token is still trained once, down to one sentence per wave and up to
+    a floor that leaves a single wavefront."""
+    vocab, stream = ragged
+    counts, T = vocab.counts(), vocab.train_words
+    csr = H.subsample(stream, counts, T, 1e-3, seed=5)
+    n = csr.sent_off.size - 1
+    print(f"[edges] floor {floor}", flush=True)
+    with _dev(vocab, "bf16", *H.init_net(len(vocab), DIM)) as dev:
+        dev.set_sampler(H.unigram_entries(counts))
+        dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
+        st = dev.train("persistent", 0, n, 0.025, T, seed=3, use_graph=True, min_sentences_per_wave=floor)
+        assert st["tokens"] == csr.tokens.size
+        with pytest.raises(DeviceError):
+            dev.train("persistent", 0, n, 0.025, T, min_sentences_per_wave=-1)
