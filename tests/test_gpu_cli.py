@@ -64,3 +64,16 @@ def test_cli_autotune(tmp_path):
     assert "launch shape (" in r.stdout and "sentences per dequeue" in r.stdout
     V, d = map(int, out.read_text().split("\n")[0].split())
     assert d == 64 and V > 1
+
+
+def test_cli_min_sentences_per_wave(tmp_path):
+    """-min-sentences-per-wave reaches the persistent grid; a bad value is an error."""
+    p = _corpus(tmp_path)
+    out = tmp_path / "vec.txt"
+    base = [str(CLI), "-train", p, "-output", str(out), "-size", "32", "-iter", "1", "-precision", "bf16",
+            "-sample", "1e-4", "-time-file", str(tmp_path / "t")]
+    r = subprocess.run(base + ["-min-sentences-per-wave", "1"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert out.read_text().split("\n")[0].split()[1] == "32"
+    r = subprocess.run(base + ["-min-sentences-per-wave", "-3"], capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "min_sentences_per_wave" in r.stderr
