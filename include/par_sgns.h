@@ -99,6 +99,20 @@ typedef struct par_sgns_stats {
     uint64_t negatives;   /* sum over batches of K */
     uint64_t rep_pairs;   /* sum over batches of K * batch rows */
     double kernel_ms;     /* device time of the training launches (HIP events) */
+    /* Algorithmic work of the pass (what the reference's per-pair and
+     * per-batch arithmetic moves, cpu:529-674, independent of caching):
+     * bytes = 4 tokens + e*dim*(4 tokens + 2 negatives), e = 4 (fp32) or
+     * 2 (bf16) -- each trained token's W_in and W_out rows read and written
+     * once, each shared negative's W_out row read and written once per batch;
+     * flops = 6*dim*(pairs + rep_pairs). */
+    double alg_bytes;
+    double alg_flops;
+    /* Launch shape used: workgroups, wavefronts per workgroup, queue slots per
+     * dequeue (plain mode: one sentence per wavefront, dequeue 0). */
+    int32_t blocks;
+    int32_t waves_per_block;
+    int32_t sentences_per_dequeue;
+    int32_t reserved;
 } par_sgns_stats;
 
 /* Handle lifetime.  Replaces the device allocations of text.cu:348-376,

@@ -53,10 +53,11 @@ class ParSgnsTunePoint(C.Structure):
 class ParSgnsStats(C.Structure):
     _fields_ = [("loss_attr", f64), ("loss_rep", f64), ("tokens", u64), ("sentences", u64),
                 ("pairs", u64), ("batches", u64), ("negatives", u64), ("rep_pairs", u64),
-                ("kernel_ms", f64)]
+                ("kernel_ms", f64), ("alg_bytes", f64), ("alg_flops", f64), ("blocks", i32),
+                ("waves_per_block", i32), ("sentences_per_dequeue", i32), ("reserved", i32)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class PhPlanInfo(C.Structure):

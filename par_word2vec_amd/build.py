@@ -4,7 +4,6 @@
                      synthetic corpora, writer)            -- g++
     libpar_sgns.so   HIP kernels for gfx950 + the device C-ABI  -- hipcc
     par_word2vec     command-line trainer (pWord2Vec-style flags)
-    atomic_probe     request-size probe of the packed-bf16 atomics -- hipcc
 
 Every call recompiles (the build check runs on a machine without a GPU;
 hipcc cross-compiles gfx950 code objects).
@@ -79,20 +78,11 @@ def build_cli(verbose: bool = False) -> Path:
     return out
 
 
-def build_probe(verbose: bool = False) -> Path:
-    """atomic_probe: request-size probe of the packed-bf16 atomic path (the
-    trainer's write-back ceiling), run under rocprofv3 for the profiles."""
-    out = LIB / "atomic_probe"
-    _run([HIPCC, *HIP_FLAGS, "-o", str(out), str(PKG / "csrc" / "device" / "atomic_probe.hip")], verbose)
-    return out
-
-
 def build_all(verbose: bool = False) -> None:
     build_host(verbose)
     build_device(verbose)
     if (PKG / "csrc" / "host" / "cli.cpp").exists():
         build_cli(verbose)
-    build_probe(verbose)
 
 
 if __name__ == "__main__":

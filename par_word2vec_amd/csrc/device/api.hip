@@ -136,6 +136,13 @@ float bf2f(uint16_t v) {
     return f;
 }
 
+// Captured graphs hold every device pointer of their launch by value, so a
+// call that frees or rewrites a buffer they read drops them first.
+void drop_graphs(par_sgns_handle h) {
+    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+}
+
 int check_ids(par_sgns_handle h, const int32_t *ids, int64_t n, int32_t lo, int32_t hi, const char *what) {
     int64_t bad = -1;
 #pragma omp parallel for reduction(max : bad)
@@ -223,6 +230,7 @@ int32_t par_sgns_row_stride(par_sgns_handle h) { return h ? h->ld : PAR_SGNS_EIN
 int par_sgns_set_exp_table(par_sgns_handle h, const float *t, int32_t n) {
     if (!h) return PAR_SGNS_EINVAL;
     if (!t || n != sgns::kExpTable + 1) return seterr(h, PAR_SGNS_EINVAL, "exp table must have 1001 entries");
+    drop_graphs(h);
     int rc = upload(h, h->exp_table, t, (size_t)n);
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->st));
@@ -278,6 +286,7 @@ int par_sgns_set_sampler(par_sgns_handle h, const int64_t *entries, int32_t V) {
         std::memcpy(&pb, &prob[(size_t)i], 4);
         tab[(size_t)i] = make_uint2(pb, (uint32_t)alias[(size_t)i]);
     }
+    drop_graphs(h);
     int rc = upload(h, h->alias_tab, tab.data(), (size_t)V);
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(h->st));
@@ -377,8 +386,7 @@ int par_sgns_set_corpus(par_sgns_handle h, const int32_t *tokens, int64_t n_toke
     h->n_tokens = n_tokens;
     h->n_sent = n_sent;
     h->ordered.clear();
-    for (auto &kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
-    h->graphs.clear();
+    drop_graphs(h);
     return 0;
 }
 
@@ -408,6 +416,7 @@ int par_sgns_set_plan(par_sgns_handle h, const int32_t *tokens, const uint8_t *b
             return seterr(h, PAR_SGNS_EINVAL, "plan batch %lld has %lld negatives", (long long)k, (long long)K);
     }
     if (neg_off[n_batches] != n_negs) return seterr(h, PAR_SGNS_EINVAL, "plan negative count mismatch");
+    drop_graphs(h);
     if ((rc = upload(h, h->p_tokens, tokens, (size_t)n_tokens))) return rc;
     if ((rc = upload(h, h->p_b, b, (size_t)n_tokens))) return rc;
     if ((rc = upload(h, h->p_sent_off, sent_off, (size_t)n_sent + 1))) return rc;
@@ -422,8 +431,24 @@ int par_sgns_set_plan(par_sgns_handle h, const int32_t *tokens, const uint8_t *b
     return 0;
 }
 
+// The queue order of a sentence range [b, e) and the progress at each of its
+// slots live in order[b..e) and queue_raw[b..e), shared by every range.
+// Ordering a range therefore invalidates each prepared range that overlaps
+// it, and the graphs captured for those ranges.
 static int ensure_order(par_sgns_handle h, int64_t b, int64_t e) {
-    if (h->ordered.count({b, e})) return 0;
+    if (e <= b || h->ordered.count({b, e})) return 0;
+    for (auto it = h->ordered.begin(); it != h->ordered.end();) {
+        if (it->first < e && b < it->second) it = h->ordered.erase(it);
+        else ++it;
+    }
+    for (auto it = h->graphs.begin(); it != h->graphs.end();) {
+        if (it->first.b < e && b < it->first.e) {
+            (void)hipGraphExecDestroy(it->second);
+            it = h->graphs.erase(it);
+        } else {
+            ++it;
+        }
+    }
     // longest sentences first within the range (balances uneven lengths)
     std::vector<int32_t> ord((size_t)(e - b));
     std::iota(ord.begin(), ord.end(), (int32_t)b);
@@ -519,6 +544,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
     int rc;
     float ms = 0.f;
     int launches = 1;
+    int shape_blocks = 1, shape_wpb = 1, shape_grab = 0;  // launch shape, reported in the stats
     if (a->mode == PAR_SGNS_DETERMINISTIC) {
         if (h->bf16) return seterr(h, PAR_SGNS_EINVAL, "deterministic mode runs on fp32 tables");
         if (!h->p_tokens) return seterr(h, PAR_SGNS_ESTATE, "no plan uploaded");
@@ -586,6 +612,9 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         if (f.grab > 64) return seterr(h, PAR_SGNS_EINVAL, "sentences_per_dequeue must be 0..64");
         if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
+        shape_wpb = wpb;
+        shape_blocks = pers ? blocks : (int)((a->sent_end - a->sent_begin + wpb - 1) / wpb);
+        shape_grab = pers ? f.grab : 0;
         if (pers && !planned && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
         if (pers && a->use_graph) {
             GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0) | (f.rep_mfma ? 4 : 0), wpb, blocks, a->iter,
@@ -639,6 +668,14 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         stats->negatives = c[sgns::kCntNegatives];
         stats->rep_pairs = c[sgns::kCntRepPairs];
         stats->kernel_ms = ms;
+        const double e = h->bf16 ? 2.0 : 4.0, dim = (double)h->cfg.dim;
+        const double tok = (double)stats->tokens, negs = (double)stats->negatives;
+        stats->alg_bytes = 4.0 * tok + e * dim * (4.0 * tok + 2.0 * negs);
+        stats->alg_flops = 6.0 * dim * ((double)stats->pairs + (double)stats->rep_pairs);
+        stats->blocks = shape_blocks;
+        stats->waves_per_block = shape_wpb;
+        stats->sentences_per_dequeue = shape_grab;
+        stats->reserved = 0;
     }
     return 0;
 }

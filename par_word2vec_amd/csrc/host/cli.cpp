@@ -197,6 +197,9 @@ int main(int argc, char **argv) {
                 ta.sent_begin = b;
                 ta.sent_end = e;
                 if (par_sgns_train(h, &ta, &st)) return fail("train", par_sgns_last_error(h));
+                if (o.debug > 1 && b == 0)
+                    std::printf("launch shape: %d blocks x %d waves, %d sentences per dequeue\n", st.blocks,
+                                st.waves_per_block, st.sentences_per_dequeue);
                 tokens += st.tokens;
                 b = e;
             }

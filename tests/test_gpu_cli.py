@@ -72,8 +72,18 @@ def test_cli_min_sentences_per_wave(tmp_path):
     out = tmp_path / "vec.txt"
     base = [str(CLI), "-train", p, "-output", str(out), "-size", "32", "-iter", "1", "-precision", "bf16",
             "-sample", "1e-4", "-time-file", str(tmp_path / "t")]
-    r = subprocess.run(base + ["-min-sentences-per-wave", "1"], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
-    assert out.read_text().split("\n")[0].split()[1] == "32"
+    shapes = {}
+    for floor in ("1", "16", "100000"):
+        r = subprocess.run(base + ["-min-sentences-per-wave", floor, "-debug", "2"], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert out.read_text().split("\n")[0].split()[1] == "32"
+        line = next(x for x in r.stdout.splitlines() if x.startswith("launch shape:"))
+        blocks, waves = int(line.split()[2]), int(line.split()[5])
+        shapes[floor] = blocks * waves
+    # the floor reached the grid: one wavefront at 100000 sentences per wave,
+    # more wavefronts at one sentence per wave than at the default 16
+    assert shapes["100000"] == 1
+    assert shapes["1"] > shapes["16"]
     r = subprocess.run(base + ["-min-sentences-per-wave", "-3"], capture_output=True, text=True, timeout=300)
     assert r.returncode != 0 and "min_sentences_per_wave" in r.stderr
