@@ -218,6 +218,22 @@ int par_sgns_loss_adjoint(par_sgns_handle h, const int32_t *centre, const int32_
                           int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out);
 int par_sgns_get_grads(par_sgns_handle h, float *g_in, float *g_out);
 
+/* par_sgns_loss_forward without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments, checks and result as
+ * par_sgns_loss_forward, but the window indices are staged and the FP64 loss
+ * is accumulated in a fixed 8 MiB device array that the library holds per
+ * device until the process ends (looked up at par_sgns_create), instead of
+ * handle buffers grown with hipMalloc.  Limit: n_windows * (1 + nctx + nneg)
+ * <= 2,097,088 indices (65,536 windows of 10 contexts and 5 negatives use
+ * 1,048,576); larger batches return PAR_SGNS_EINVAL.  Every handle on one
+ * device shares that array and it keeps the last call's contents (the loss
+ * accumulators are cleared by each call): calls that overlap on one device
+ * (handles on different host threads or streams) race and must be
+ * serialised by the caller. */
+int par_sgns_loss_forward_noalloc(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
+                                  int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out);
+
 /* Device time (ms) and launch count of the most recent kernel of the given
  * kind: 0 = training, 1 = loss forward, 2 = adjoint. */
 int par_sgns_last_kernel_time(par_sgns_handle h, int32_t kind, double *ms, int32_t *launches);

@@ -114,6 +114,7 @@ SGNS_SYMBOLS = {
     "par_sgns_set_launch_shape": (C.c_int, [vp, i32, i32, i32]),
     "par_sgns_loss_forward": (C.c_int, [vp, vp, vp, vp, i64, i32, i32, i32, vp]),
     "par_sgns_loss_adjoint": (C.c_int, [vp, vp, vp, vp, i64, i32, i32, i32, vp]),
+    "par_sgns_loss_forward_noalloc": (C.c_int, [vp, vp, vp, vp, i64, i32, i32, i32, vp]),
     "par_sgns_get_grads": (C.c_int, [vp, vp, vp]),
     "par_sgns_last_kernel_time": (C.c_int, [vp, i32, vp, vp]),
 }

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <string>
@@ -35,7 +36,20 @@ struct GraphKey {
     }
 };
 
+// Device scratch of the allocation-free window loss (par_sgns_loss_forward_noalloc):
+// the FP64 loss accumulators in the first 256 bytes, then the staged window
+// indices.  Each device holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name is a host-side stand-in).
+constexpr size_t kScratchBytes = size_t(8) << 20;
+constexpr size_t kScratchIdx = 256;  // byte offset of the staged indices
+
+std::mutex g_scratch_mu;
+std::vector<char *> g_scratch_ptr;  // per device ordinal, looked up once
+
 }  // namespace
+
+static __device__ __align__(256) char scratch[kScratchBytes];
 
 struct par_sgns_handle_s {
     par_sgns_config cfg{};
@@ -70,6 +84,7 @@ struct par_sgns_handle_s {
     double *loss = nullptr;
     // window kernels
     float *g_in = nullptr, *g_out = nullptr;
+    char *scratch = nullptr;  // this device's static scratch (device address)
     int32_t *win_buf = nullptr;
     size_t win_cap = 0;
     std::map<GraphKey, hipGraphExec_t> graphs;
@@ -143,6 +158,23 @@ void drop_graphs(par_sgns_handle h) {
     h->graphs.clear();
 }
 
+// Device address of `scratch` on the current device `dev` (looked up once per
+// device, with that device current).
+int scratch_address(par_sgns_handle h, int dev, char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[(size_t)dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return seterr(h, PAR_SGNS_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                          hipGetErrorString(e));
+        g_scratch_ptr[(size_t)dev] = static_cast<char *>(p);
+    }
+    *out = g_scratch_ptr[(size_t)dev];
+    return 0;
+}
+
 int check_ids(par_sgns_handle h, const int32_t *ids, int64_t n, int32_t lo, int32_t hi, const char *what) {
     int64_t bad = -1;
 #pragma omp parallel for reduction(max : bad)
@@ -192,6 +224,8 @@ int par_sgns_create(const par_sgns_config *cfg, par_sgns_handle *out) {
     HIPCHK(h, hipEventCreate(&h->ev0));
     HIPCHK(h, hipEventCreate(&h->ev1));
     HIPCHK(h, hipDeviceGetAttribute(&h->cu_count, hipDeviceAttributeMultiprocessorCount, cfg->device));
+    int rc0 = scratch_address(h, cfg->device, &h->scratch);
+    if (rc0) return rc0;
     const size_t rows = (size_t)cfg->vocab_size * (size_t)h->ld;
     const size_t esz = h->bf16 ? 2 : 4;
     int rc;
@@ -783,8 +817,10 @@ int par_sgns_set_launch_shape(par_sgns_handle h, int32_t wpb, int32_t wpcu, int3
     return 0;
 }
 
+// noalloc: stage the indices and accumulate the loss in the device's static
+// scratch instead of the handle's buffers (no allocation on this path).
 static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg, int64_t nw,
-                       int32_t nctx, int32_t nneg, int32_t flags, double *loss_out, int adjoint) {
+                       int32_t nctx, int32_t nneg, int32_t flags, double *loss_out, int adjoint, bool noalloc = false) {
     if (!h) return PAR_SGNS_EINVAL;
     if (!centre || !ctx || (nneg > 0 && !neg) || nw < 0 || nctx < 1 || nctx > 32 || nneg < 0 || nneg > 7)
         return seterr(h, PAR_SGNS_EINVAL, "bad window batch (nctx 1..32, nneg 0..7)");
@@ -799,11 +835,25 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
     if ((rc = check_ids(h, neg, nw * nneg, 0, V, "neg"))) return rc;
     HIPCHK(h, hipSetDevice(h->cfg.device));
     const size_t need = (size_t)nw * (size_t)(1 + nctx + nneg);
-    if (need > h->win_cap) {
-        if ((rc = dalloc(h, h->win_buf, need))) return rc;
-        h->win_cap = need;
+    int32_t *dc = nullptr;
+    double *dloss = h->loss;
+    if (noalloc) {
+        if (need > (kScratchBytes - kScratchIdx) / sizeof(int32_t))
+            return seterr(h, PAR_SGNS_EINVAL,
+                          "window batch needs %zu bytes of indices; the allocation-free entry point stages at most "
+                          "%zu (n_windows * (1 + nctx + nneg) <= %zu)",
+                          need * sizeof(int32_t), kScratchBytes - kScratchIdx,
+                          (kScratchBytes - kScratchIdx) / sizeof(int32_t));
+        dc = reinterpret_cast<int32_t *>(h->scratch + kScratchIdx);
+        dloss = reinterpret_cast<double *>(h->scratch);
+    } else {
+        if (need > h->win_cap) {
+            if ((rc = dalloc(h, h->win_buf, need))) return rc;
+            h->win_cap = need;
+        }
+        dc = h->win_buf;
     }
-    int32_t *dc = h->win_buf, *dx = dc + nw, *dn = dx + nw * nctx;
+    int32_t *dx = dc + nw, *dn = dx + nw * nctx;
     HIPCHK(h, hipMemcpyAsync(dc, centre, (size_t)nw * 4, hipMemcpyHostToDevice, h->st));
     HIPCHK(h, hipMemcpyAsync(dx, ctx, (size_t)nw * nctx * 4, hipMemcpyHostToDevice, h->st));
     if (nneg > 0) HIPCHK(h, hipMemcpyAsync(dn, neg, (size_t)nw * nneg * 4, hipMemcpyHostToDevice, h->st));
@@ -816,11 +866,13 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
         HIPCHK(h, hipMemsetAsync(h->g_in, 0, rows * 4, h->st));
         HIPCHK(h, hipMemsetAsync(h->g_out, 0, rows * 4, h->st));
     }
-    HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, nullptr, h->st));
+    // loss accumulators cleared by a kernel (no memset node if a caller captures this)
+    if (noalloc) HIPCHK(h, sgns::launch_reset(nullptr, 0, dloss, nullptr, h->st));
+    else HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, nullptr, h->st));
     sgns::WindowArgs w{};
     w.centre = dc; w.ctx = dx; w.neg = dn; w.n_windows = nw; w.nctx = nctx; w.nneg = nneg;
     w.d = h->cfg.dim; w.ld = h->ld; w.w_in = h->w_in; w.w_out = h->w_out; w.g_in = h->g_in; w.g_out = h->g_out;
-    w.loss = h->loss;
+    w.loss = dloss;
     HIPCHK(h, hipEventRecord(h->ev0, h->st));
     hipError_t e = sgns::launch_window(w, h->bf16, h->np, adjoint, flags, h->st);
     if (e == hipErrorNotSupported) return seterr(h, PAR_SGNS_EINVAL, "requested window path is not built");
@@ -832,9 +884,14 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
     h->last_ms[adjoint ? 2 : 1] = ms;
     h->last_launches[adjoint ? 2 : 1] = 1;
     double l[2];
-    HIPCHK(h, hipMemcpy(l, h->loss, sizeof l, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(l, dloss, sizeof l, hipMemcpyDeviceToHost));
     if (loss_out) *loss_out = l[0];
     return 0;
+}
+
+int par_sgns_loss_forward_noalloc(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
+                                  int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out) {
+    return window_call(h, centre, ctx, neg, n_windows, nctx, nneg, flags, loss_out, 0, true);
 }
 
 int par_sgns_loss_forward(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,

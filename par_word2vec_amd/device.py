@@ -159,6 +159,11 @@ class SgnsDevice:
     def loss_forward(self, centre, ctx, neg, flags: int = 0) -> float:
         return self._windows(self.lib.par_sgns_loss_forward, centre, ctx, neg, flags)
 
+    def loss_forward_noalloc(self, centre, ctx, neg, flags: int = 0) -> float:
+        """loss_forward staged in the library's fixed per-device scratch (no
+        device allocation; at most 2,097,088 window indices per call)."""
+        return self._windows(self.lib.par_sgns_loss_forward_noalloc, centre, ctx, neg, flags)
+
     def loss_adjoint(self, centre, ctx, neg, flags: int = 0) -> float:
         return self._windows(self.lib.par_sgns_loss_adjoint, centre, ctx, neg, flags)
 
