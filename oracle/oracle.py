@@ -59,6 +59,8 @@ def lib(flavour: str = "parity") -> C.CDLL:
     L = C.CDLL(str(path))
     L.orc_lcg.restype = u64
     L.orc_lcg.argtypes = [u64]
+    L.orc_set_mutation.argtypes = [i32]
+    L.orc_mutation.restype = i32
     L.orc_exp_resolution.restype = f32
     L.orc_exp_table.argtypes = [vp]
     L.orc_init_net.argtypes = [i64, i32, vp, vp]
@@ -94,6 +96,27 @@ def lib(flavour: str = "parity") -> C.CDLL:
     L.orc_window_grad_f64.argtypes = [i64, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp]
     _libs[name] = L
     return L
+
+
+# Planted misreadings (ORC_MUT_* in par_sgns_oracle.h), for the tests that
+# show each known-answer pin fails under the misreading it guards against.
+MUTATIONS = {"alpha_after_load": 1, "alpha_ge": 2, "no_rejection": 3, "repb_2w": 4, "window_incl": 5,
+             "pair_order": 6, "unigram_linear": 7, "subsample_flip": 8, "table_index": 9, "wca_local": 10,
+             "rep_label": 11, "batch_end": 12}
+
+
+class mutation:
+    """Context manager: run the parity oracle with one planted misreading."""
+
+    def __init__(self, name: str):
+        self.m = MUTATIONS[name]
+
+    def __enter__(self):
+        lib().orc_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_mutation(0)
 
 
 def P(a: np.ndarray | None):

@@ -28,6 +28,15 @@
 
 #define ORC_HASH_SIZE 30000000 /* cpu:74 */
 
+/* ---------------------------------------------------- planted misreadings */
+
+/* 0 in every real use.  The pin tests set one ORC_MUT_* value at a time to
+ * show that the oracle's known-answer checks fail under that misreading. */
+static int g_mutation = 0;
+void orc_set_mutation(int32_t m) { g_mutation = m; }
+int32_t orc_mutation(void) { return g_mutation; }
+#define MUT(k) (g_mutation == (k))
+
 /* ---------------------------------------------------------------- LCG ---- */
 
 /* next_random = next_random * 25214903917 + 11 (mod 2^64), cpu:461, 532, 591 */
@@ -65,7 +74,7 @@ void orc_init_net(int64_t vocab_size, int32_t dim, float *w_in, float *w_out) {
  * that read never influences the table, so it is skipped here. */
 int orc_unigram_table_n(const uint32_t *cn, int32_t vocab_size, int32_t *table, int64_t table_size) {
     if (vocab_size <= 0) return -1;
-    const float power = 0.75f;
+    const float power = MUT(ORC_MUT_UNIGRAM_LINEAR) ? 1.0f : 0.75f;
     double z = 0.0;
     for (int32_t i = 0; i < vocab_size; i++) z += pow((double)cn[i], (double)power);
     int32_t i = 0;
@@ -434,7 +443,8 @@ static void orc_thread_run(orc_thread *t) {
 
     for (;;) {
         /* cpu:431-446 learning-rate update, checked before loading */
-        if (word_count - last_word_count > 10000) {
+        if (!MUT(ORC_MUT_ALPHA_AFTER_LOAD) &&
+            (word_count - last_word_count > 10000 || (MUT(ORC_MUT_ALPHA_GE) && word_count - last_word_count >= 10000))) {
             uint64_t diff = word_count - last_word_count;
             uint64_t wca;
             if (t->shared) {
@@ -460,7 +470,9 @@ static void orc_thread_run(orc_thread *t) {
                     float ran = sqrtf(ratio) + ratio;
                     x = orc_lcg(x);
                     t->st.draws++;
-                    if (ran < (float)(x & 0xFFFF) / 65536.f) continue;
+                    if (MUT(ORC_MUT_SUBSAMPLE_FLIP) ? !(ran < (float)(x & 0xFFFF) / 65536.f)
+                                                    : ran < (float)(x & 0xFFFF) / 65536.f)
+                        continue;
                 }
                 sen[sentence_length] = w;
                 sentence_length++;
@@ -470,9 +482,15 @@ static void orc_thread_run(orc_thread *t) {
             rep_sentence_position = 0;
             sent_trace_idx = -1;
         }
+        if (MUT(ORC_MUT_ALPHA_AFTER_LOAD) && word_count - last_word_count > 10000) {
+            *t->wc_actual += word_count - last_word_count;
+            last_word_count = word_count;
+            alpha = starting_alpha * (1 - (float)*t->wc_actual / (float)denom_words);
+            if (alpha < starting_alpha * 0.0001f) alpha = starting_alpha * 0.0001f;
+        }
         /* cpu:472-527 epoch boundary: the sentence just built is dropped */
         if (word_count > t->local_words) {
-            uint64_t diff = word_count - last_word_count;
+            uint64_t diff = (MUT(ORC_MUT_WCA_LOCAL) ? t->local_words : word_count) - last_word_count;
             if (t->shared) {
 #pragma omp atomic
                 *t->wc_actual += diff;
@@ -506,7 +524,7 @@ static void orc_thread_run(orc_thread *t) {
             t->st.trained_tokens++;
             if (t->tr && sent_trace_idx >= 0) t->tr->b[t->tr->n_tokens + sentence_position] = (uint8_t)b;
         }
-        for (int i = b; i < 2 * W + 1 - b; i++) {
+        for (int i = b; i < 2 * W + 1 - b + (MUT(ORC_MUT_WINDOW_INCL) ? 1 : 0); i++) {
             if (i == W) continue;
             int c = sentence_position - W + i;
             if (c < 0) continue;
@@ -522,6 +540,7 @@ static void orc_thread_run(orc_thread *t) {
             for (int j = 0; j < d; j++) {
                 float tmp = g * w_out[l2 + j];
                 w_out[l2 + j] += g * w_in[l1 + j];
+                if (MUT(ORC_MUT_PAIR_ORDER)) tmp = g * w_out[l2 + j];
                 w_in[l1 + j] += tmp;
             }
             t->st.pairs++;
@@ -547,16 +566,19 @@ static void orc_thread_run(orc_thread *t) {
                     int rep_b;
                     x = orc_lcg(x);
                     t->st.draws++;
-                    if (min_pos <= W - 1) rep_b = (int)(x % (uint64_t)W);
-                    else if ((sentence_length - 1) - min_pos <= W - 1) rep_b = (int)(x % (uint64_t)W);
+                    const int end_gap = MUT(ORC_MUT_BATCH_END) ? sentence_length - min_pos : (sentence_length - 1) - min_pos;
+                    if (MUT(ORC_MUT_REPB_2W)) rep_b = (int)(x % (uint64_t)(W * 2));
+                    else if (min_pos <= W - 1) rep_b = (int)(x % (uint64_t)W);
+                    else if (end_gap <= W - 1) rep_b = (int)(x % (uint64_t)W);
                     else rep_b = (int)(x % (uint64_t)(W * 2));
                     int K = rep_b * NEG;
                     for (int ng = 0; ng < K; ng++) {
                         x = orc_lcg(x);
                         t->st.draws++;
-                        int s = t->table[(x >> 16) % (uint64_t)ORC_TABLE_SIZE];
+                        int s = t->table[(MUT(ORC_MUT_TABLE_INDEX) ? x : x >> 16) % (uint64_t)ORC_TABLE_SIZE];
                         if (!s) s = (int)(x % (uint64_t)(t->V - 1) + 1);
                         for (;;) {
+                            if (MUT(ORC_MUT_NO_REJECTION)) break;
                             int found = 0;
                             for (int q = 0; q < sentence_length; q++)
                                 if (sen[q] == s) { found = 1; break; }
@@ -593,9 +615,10 @@ static void orc_thread_run(orc_thread *t) {
                                 int64_t k = (int64_t)(sent_key / (uint64_t)t->cv->bucket_words);
                                 if (k < t->cv->n_buckets) { t->cv->loss[k] += lp; t->cv->pairs[k] += 1; }
                             }
-                            if (f > ORC_MAX_EXP) f = (0 - 1) * alpha;
-                            else if (f < -ORC_MAX_EXP) f = 0 * alpha;
-                            else f = (0 - et[(int)((f + ORC_MAX_EXP) * EXP_RES)]) * alpha;
+                            const int label = MUT(ORC_MUT_REP_LABEL) ? 1 : 0;
+                            if (f > ORC_MAX_EXP) f = (label - 1) * alpha;
+                            else if (f < -ORC_MAX_EXP) f = label * alpha;
+                            else f = (label - et[(int)((f + ORC_MAX_EXP) * EXP_RES)]) * alpha;
                             M_grad[i * input_size + j] = f;
                         }
                     }

@@ -82,6 +82,27 @@ typedef struct orc_curve {
     double *pairs;
 } orc_curve;
 
+/* Planted misreadings of the reference (test infrastructure for the
+ * infrastructure): the pin tests switch one on at a time and check that the
+ * oracle's known answers then fail.  0 (none) in every parity use. */
+enum {
+    ORC_MUT_NONE = 0,
+    ORC_MUT_ALPHA_AFTER_LOAD = 1, /* learning-rate check after the sentence load (cpu:431-446 is before it) */
+    ORC_MUT_ALPHA_GE = 2,         /* ">=" in the 10000-word check (cpu:431 has ">") */
+    ORC_MUT_NO_REJECTION = 3,     /* no in-sentence rejection of negatives (cpu:609-617) */
+    ORC_MUT_REPB_2W = 4,          /* rep_b = x % 2W for every batch (cpu:590-601: x % W at the ends) */
+    ORC_MUT_WINDOW_INCL = 5,      /* window loop to 2W+1-b inclusive (cpu:536 stops before) */
+    ORC_MUT_PAIR_ORDER = 6,       /* W_in takes g * the updated W_out (cpu:562-564 saves tmp first) */
+    ORC_MUT_UNIGRAM_LINEAR = 7,   /* unigram table without the 0.75 power (cpu:98-112) */
+    ORC_MUT_SUBSAMPLE_FLIP = 8,   /* subsampling keeps exactly the words the reference drops (cpu:462-463) */
+    ORC_MUT_TABLE_INDEX = 9,      /* table slot x % size instead of (x >> 16) % size (cpu:606) */
+    ORC_MUT_WCA_LOCAL = 10,       /* epoch end adds local_words - last, not word_count - last (cpu:515-516) */
+    ORC_MUT_REP_LABEL = 11,       /* repulsion gradient with label 1 instead of 0 (cpu:641-647) */
+    ORC_MUT_BATCH_END = 12,       /* end-of-sentence batch rule with L - min_pos (cpu:594 has (L-1) - min_pos) */
+};
+void orc_set_mutation(int32_t m);
+int32_t orc_mutation(void);
+
 uint64_t orc_lcg(uint64_t x);                                   /* cpu:461 */
 float orc_exp_resolution(void);                                 /* cpu:70 */
 void orc_exp_table(float *table_1001);                          /* cpu:822-826 */

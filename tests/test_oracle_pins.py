@@ -12,6 +12,10 @@ checked against something the reference or mathematics fixes:
   whole trainer       an independent pure-Python float32 transcription of
                       cpu:430-685 on tiny corpora, bit for bit
   window loss/grad    numpy float64 and central finite differences
+
+The training schedule itself (RNG interleaving, learning-rate timing, batch
+rules, rejection, the dropped tail) is pinned by hand-derived known answers in
+test_oracle_known_answers.py, which also shows each planted misreading fails.
 """
 import math
 
