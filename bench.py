@@ -16,7 +16,10 @@ Both variants train the same epoch from the same InitNet weights:
                   before the warmup)
 `value` is the bf16 persistent rate in Mwords/s: raw corpus words (every
 token incl. those removed by subsampling and "</s>") per second of the timed
-steps.  The planted-cluster recall@10 of both runs is reported beside it.
+steps.  Each variant's tables are scored after the full epoch: planted-cluster
+recall@10, the planted-cluster cosine margin (does not saturate) and the FP64
+SGNS loss per pair of 65,536 held-out windows (Zipf workloads), or community
+kNN accuracy (SBM walks).
 
 The CPU baseline is the oracle's OpenMP Hogwild restatement of the
 reference CPU trainer, timed on a bounded prefix of the same corpus."""
@@ -34,7 +37,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-from par_word2vec_amd import autotune, corpora, host as H  # noqa: E402
+from par_word2vec_amd import autotune, corpora, host as H, profiling  # noqa: E402
 from par_word2vec_amd.device import SgnsDevice  # noqa: E402
 from par_word2vec_amd.trainer import chunk_bounds  # noqa: E402
 
@@ -113,6 +116,7 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     elem = 2 if precision == "bf16" else 4
     byts = algorithmic_bytes(tokens, negs, w.dim, elem)
     kind = "matrix" if precision == "bf16" and repulsion != "vector" else "vector"  # kernel instance
+    shape = stats[-1] if stats else {}
     # the same rate when the epoch's corpus upload (host buffers across PCIe) is charged pro rata
     pcie_s = wall + upload_s * raw / max(int(c.stream.size), 1)
     out = dict(mode=mode, precision=precision, kernel=f"{precision}-{kind}", wall_s=wall, upload_s=upload_s,
@@ -120,11 +124,11 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
                negatives=negs, pairs=sum(s["pairs"] for s in stats), mwords_s=raw / wall / 1e6,
                kernel_mwords_s=raw / (kern_ms / 1e3) / 1e6, alg_bytes=byts, launches=len(stats),
                avg_launch_ms=kern_ms / max(len(stats), 1), achieved_gbs=byts / (kern_ms / 1e3) / 1e9)
+    out["grid"] = {k: shape.get(k) for k in ("blocks", "waves_per_block", "sentences_per_dequeue")}
     if mode == "persistent":
         out["launch"] = launch or {"source": "built-in default (most resident waves, one sentence per dequeue)"}
     if not profile_only:
-        g_in, _ = dev.get_weights()
-        out["w_in"] = g_in
+        out["w_in"], out["w_out"] = dev.get_weights()
     dev.close()
     log(f"{mode}/{precision}: {out['mwords_s']:.1f} Mwords/s wall, {out['kernel_mwords_s']:.1f} kernel, "
         f"{out['achieved_gbs']:.0f} GB/s algorithmic")
@@ -148,21 +152,32 @@ def calibration_copy(mib: int) -> None:
 def measured_traffic(variant: str) -> float | None:
     """HBM bytes per launch of the training kernel from the committed counter
     profile of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
-    calibration copy), or None."""
+    calibration copy), or None -- also when the profile was taken on a
+    different build of the kernel's sources."""
     try:
         t = json.loads(TRAFFIC_FILE.read_text())
+        if t.get("kernel_source") != profiling.kernel_source_digest():
+            log("counter profile is of another kernel build: traffic not reported")
+            return None
         return float(t["variants"][variant]["bytes_per_launch"])
     except (OSError, KeyError, ValueError, TypeError):
         return None
 
 
-def cluster_score(c, w_in) -> float | None:
+def quality(c, w_in, w_out, windows) -> dict:
+    """Scores of one variant's tables after the epoch (see the module doc)."""
     from par_word2vec_amd import evaluate as E
+    out = {}
     try:
-        return E.planted_cluster_recall(w_in, c.groups)
+        if c.workload.kind == "sbm":
+            out["community_knn_accuracy"] = round(E.community_knn_accuracy(w_in, c.labels), 5)
+        else:
+            out["recall_at_10"] = round(E.planted_cluster_recall(w_in, c.groups), 5)
+            out["cluster_margin"] = round(E.cluster_margin(w_in, c.groups), 5)
+            out["heldout_loss_per_pair"] = round(E.heldout_loss(w_in, w_out, *windows), 6)
     except Exception as ex:  # noqa: BLE001 - scoring must not hide the throughput line
-        log(f"cluster scoring failed: {ex!r}")
-        return None
+        log(f"scoring failed: {ex!r}")
+    return out
 
 
 def cpu_baseline(c, csr, seconds: float) -> dict:
@@ -234,9 +249,15 @@ def main() -> int:
                                  workload=args.workload, tune=args.autotune)
     scores = {}
     if not args.no_score:
+        windows = None
+        if w.kind == "zipf":
+            from par_word2vec_amd import evaluate as E
+            held = corpora.heldout_stream(c, 3_000_000, seed=args.seed + 98)
+            windows = E.heldout_windows(held, c.counts, c.train_words, w.sample, w.window, 5, 65_536,
+                                        seed=args.seed + 7)
         for v, r in results.items():
-            scores[v] = cluster_score(c, r.pop("w_in"))
-            log(f"{v}: planted-cluster recall@10 = {scores[v]}")
+            scores[v] = quality(c, r.pop("w_in"), r.pop("w_out"), windows)
+            log(f"{v}: {scores[v]}")
     head = results.get("bf16-persistent") or next(iter(results.values()))
     base = results.get("fp32-plain")
     cpu = None if args.no_cpu else cpu_baseline(c, csr, args.cpu_seconds)
@@ -259,7 +280,7 @@ def main() -> int:
                    "variant": f"{head['precision']} {head['mode']}", "parallelism": "single GPU"},
         "fp32_plain_mwords_s": round(base["mwords_s"], 2) if base else None,
         "speedup_vs_fp32_plain": round(head["mwords_s"] / base["mwords_s"], 3) if base else None,
-        "cluster_recall_at_10": {k: (round(s, 4) if s is not None else None) for k, s in scores.items()},
+        "quality": scores,
         "variants": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
                      for k, r in results.items()},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,

@@ -62,6 +62,7 @@ class Corpus:
     train_words: int
     groups: np.ndarray         # vocab ids: clusters [C, m] (zipf) or -1 padded
     labels: np.ndarray | None  # sbm: community of each vocab id (-1 for "</s>")
+    raw_clusters: np.ndarray | None = None  # zipf: raw ids of the planted clusters [C, m]
 
 
 def make(name: str, scale: float = 1.0) -> Corpus:
@@ -90,4 +91,54 @@ def make(name: str, scale: float = 1.0) -> Corpus:
         ok = ids >= 0
         labels[ids[ok]] = comm[ok]
         groups = np.zeros((0, 0), np.int32)
-    return Corpus(w, raw, raw_vocab, vocab, stream, vocab.counts(), vocab.train_words, groups, labels)
+    return Corpus(w, raw, raw_vocab, vocab, stream, vocab.counts(), vocab.train_words, groups, labels,
+                  clusters if w.kind == "zipf" else None)
+
+
+def heldout_stream(c: Corpus, n_tokens: int, seed: int = 99) -> np.ndarray:
+    """Held-out lines of a Zipf workload: fresh tokens from the same rank
+    distribution, line lengths and planted clusters as the training corpus,
+    mapped through the training vocabulary (words it dropped are skipped).
+    The generator's replacement rule -- each token of a clustered word
+    replaces each neighbour within +-span on its line by a random word of its
+    cluster with probability p -- is applied one offset at a time over all
+    tokens.  Returns vocabulary ids, 0 ending each line."""
+    w = c.workload
+    if w.kind != "zipf":
+        raise ValueError("held-out lines are defined for the Zipf workloads")
+    g = w.gen
+    rng = np.random.default_rng(seed)
+    V = g["raw_vocab"]
+    cdf = np.cumsum(np.arange(1, V + 1, dtype=np.float64) ** -g["zipf_s"])
+    cdf /= cdf[-1]
+    lens = []
+    while sum(lens) < n_tokens:
+        if "lognorm" in g:
+            mu, sig = g["lognorm"]
+            ln = np.clip(np.round(np.exp(mu + sig * rng.standard_normal(1024))), *g["len_clip"]).astype(np.int64)
+        else:
+            ln = np.full(1024, g["line_len"], np.int64)
+        lens.extend(ln.tolist())
+    lens = np.array(lens, np.int64)
+    lens = lens[: int(np.searchsorted(np.cumsum(lens), n_tokens)) + 1]
+    n = int(lens.sum())
+    orig = np.minimum(np.searchsorted(cdf, rng.random(n)), V - 1) + 1
+    line_of = np.repeat(np.arange(lens.size), lens)
+    cl = c.raw_clusters
+    owner = np.full(V + 1, -1, np.int64)
+    owner[cl.reshape(-1)] = np.repeat(np.arange(cl.shape[0]), cl.shape[1])
+    tok = orig.copy()
+    src = np.flatnonzero(owner[orig] >= 0)
+    for o in range(-g["span"], g["span"] + 1):
+        if o == 0:
+            continue
+        j = src + o
+        ok = (j >= 0) & (j < n)
+        ok[ok] &= line_of[j[ok]] == line_of[src[ok]]
+        ok &= rng.random(src.size) < g["p_replace"]
+        tok[j[ok]] = cl[owner[orig[src[ok]]], rng.integers(0, cl.shape[1], int(ok.sum()))]
+    ends = np.cumsum(lens) - 1
+    raw = np.insert(tok, ends + 1, 0)  # a 0 after each line
+    rmap = c.vocab.raw_map()
+    ids = np.where(raw == 0, 0, rmap[raw])
+    return ids[ids >= 0].astype(np.int32)

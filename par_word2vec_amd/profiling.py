@@ -14,11 +14,24 @@ variant."""
 from __future__ import annotations
 
 import csv
+import hashlib
 import json
 import sys
 from collections import defaultdict
+from pathlib import Path
 
 COPY_KERNEL = "neg_kernel"  # torch.neg of the calibration buffer (bench.py --calibrate-copy-mb)
+KERNEL_SOURCES = ("fast_kernel.hip", "sgns_device.h", "sgns_internal.h")
+
+
+def kernel_source_digest() -> str:
+    """SHA-256 over the training kernel's source files: a counter profile is
+    valid only for the kernel build it was taken on."""
+    h = hashlib.sha256()
+    d = Path(__file__).resolve().parent / "csrc" / "device"
+    for name in KERNEL_SOURCES:
+        h.update((d / name).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def variant_of(kernel_name: str) -> str | None:
@@ -51,7 +64,7 @@ def counter_means(path: str, counter: str) -> dict[str, tuple[int, float]]:
 
 def traffic(fetch_csv: str, write_csv: str, copy_mib: float) -> dict:
     copy_bytes = copy_mib * (1 << 20)
-    out = {"copy_bytes": copy_bytes, "counters": {}, "variants": {}}
+    out = {"kernel_source": kernel_source_digest(), "copy_bytes": copy_bytes, "counters": {}, "variants": {}}
     per_kernel: dict[str, dict] = defaultdict(dict)
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         means = counter_means(path, name)
