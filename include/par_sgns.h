@@ -47,6 +47,25 @@ enum par_sgns_rounding { PAR_SGNS_ROUND_STOCHASTIC = 0, PAR_SGNS_ROUND_NEAREST =
  * products.  VECTOR: fp32 dot products, register blocks of negatives. */
 enum par_sgns_repulsion { PAR_SGNS_REP_AUTO = 0, PAR_SGNS_REP_VECTOR = 1, PAR_SGNS_REP_MATRIX = 2 };
 
+/* How the matrix-core instance writes a row back.  ATOMIC: a packed-bf16
+ * atomic adds the step from the row as the wavefront read it to its rounded
+ * new value, so a concurrent update is never lost.  The others store the
+ * rounded value with plain stores, like the reference CPU trainer's in-place
+ * Hogwild updates (cpu:561-565, 658-674): a concurrent update of the row
+ * between the wavefront's read and its store is overwritten.  STORE_OUT
+ * stores W_out rows (centres and shared negatives, held for one centre or one
+ * batch) and keeps atomics for W_in rows (held in the window ring for 2W+1
+ * centres, so concurrent updates of a frequent word's row are common there);
+ * STORE_IN the reverse; STORE stores both tables; HOT stores rows with
+ * id >= hot_rows (the less frequent words) of both tables. */
+enum par_sgns_write_back {
+    PAR_SGNS_WB_ATOMIC = 0,
+    PAR_SGNS_WB_STORE_OUT = 1,
+    PAR_SGNS_WB_STORE_IN = 2,
+    PAR_SGNS_WB_STORE = 3,
+    PAR_SGNS_WB_HOT = 4,
+};
+
 enum par_sgns_mode {
     PAR_SGNS_DETERMINISTIC = 0, /* one wavefront replays an uploaded plan exactly */
     PAR_SGNS_PLAIN = 1,         /* one workgroup (one wavefront) per sentence */
@@ -87,6 +106,9 @@ typedef struct par_sgns_train_args {
     int32_t min_sentences_per_wave; /* persistent mode, default grid: at least this many
                                        sentences per wavefront (fewer waves on small ranges:
                                        Hogwild staleness); 0 = 16 */
+    int32_t write_back;   /* par_sgns_write_back; the vector repulsion (fp32 tables, or
+                             bf16 on request) accepts ATOMIC and STORE_OUT */
+    int32_t hot_rows;     /* PAR_SGNS_WB_HOT: rows with id < hot_rows keep their atomics */
 } par_sgns_train_args;
 
 typedef struct par_sgns_stats {

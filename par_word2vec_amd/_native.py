@@ -39,7 +39,8 @@ class ParSgnsTrainArgs(C.Structure):
                 ("alpha", f32), ("waves_per_block", i32), ("blocks", i32), ("use_graph", i32),
                 ("schedule", i32), ("repulsion", i32),
                 ("train_words", u64), ("seed", u64), ("sent_begin", i64), ("sent_end", i64),
-                ("sentences_per_dequeue", i32), ("min_sentences_per_wave", i32)]
+                ("sentences_per_dequeue", i32), ("min_sentences_per_wave", i32), ("write_back", i32),
+                ("hot_rows", i32)]
 
 
 class ParSgnsTunePoint(C.Structure):

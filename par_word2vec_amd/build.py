@@ -14,6 +14,7 @@ import os
 import shlex
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -34,6 +35,12 @@ HIP_FLAGS = [
 DEVICE_SOURCES = [
     ("det_kernel.hip", ["-ffp-contract=off"]),
     ("fast_kernel.hip", []),
+    ("fast_inst_vec_fp32.hip", []),
+    ("fast_inst_vec_bf16.hip", []),
+    ("fast_inst_mfma_atomic.hip", []),
+    ("fast_inst_mfma_store_out.hip", []),
+    ("fast_inst_mfma_store.hip", []),
+    ("fast_inst_mfma_hot.hip", []),
     ("window_kernel.hip", []),
     ("api.hip", ["-fopenmp"]),
 ]
@@ -59,11 +66,15 @@ def build_host(verbose: bool = False) -> Path:
 def build_device(verbose: bool = False) -> Path:
     LIB.mkdir(exist_ok=True)
     OBJ.mkdir(exist_ok=True)
-    objs = []
-    for src, extra in DEVICE_SOURCES:
+    def compile_one(item):
+        src, extra = item
         o = OBJ / (Path(src).stem + ".o")
         _run([HIPCC, *HIP_FLAGS, *extra, "-c", str(PKG / "csrc" / "device" / src), "-o", str(o)], verbose)
-        objs.append(str(o))
+        return str(o)
+
+    # translation units compile in parallel (the training kernel's instances are split over several)
+    with ThreadPoolExecutor(max_workers=min(len(DEVICE_SOURCES), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(compile_one, DEVICE_SOURCES))
     out = LIB / "libpar_sgns.so"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", str(out), *objs], verbose)
     return out

@@ -30,9 +30,11 @@ struct GraphKey {
     int32_t epoch, loss, wpb, blocks, iter, grab;
     uint32_t alpha_bits;
     uint64_t seed, T;
+    uint32_t hot_bytes;
     bool operator<(const GraphKey &o) const {
-        return std::tie(b, e, epoch, loss, wpb, blocks, iter, grab, alpha_bits, seed, T) <
-               std::tie(o.b, o.e, o.epoch, o.loss, o.wpb, o.blocks, o.iter, o.grab, o.alpha_bits, o.seed, o.T);
+        return std::tie(b, e, epoch, loss, wpb, blocks, iter, grab, alpha_bits, seed, T, hot_bytes) <
+               std::tie(o.b, o.e, o.epoch, o.loss, o.wpb, o.blocks, o.iter, o.grab, o.alpha_bits, o.seed, o.T,
+                        o.hot_bytes);
     }
 };
 
@@ -631,6 +633,15 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         if (a->repulsion == PAR_SGNS_REP_MATRIX && (!h->bf16 || !mfma_fits(h)))
             return seterr(h, PAR_SGNS_EINVAL, "matrix-core repulsion runs on bf16 tables below 4 GiB with batch size <= 32");
         f.rep_mfma = use_mfma(h, a->repulsion);
+        if (a->write_back < PAR_SGNS_WB_ATOMIC || a->write_back > PAR_SGNS_WB_HOT)
+            return seterr(h, PAR_SGNS_EINVAL, "write_back must be ATOMIC, STORE_OUT, STORE_IN, STORE or HOT");
+        if (a->write_back != PAR_SGNS_WB_ATOMIC && a->write_back != PAR_SGNS_WB_STORE_OUT && !f.rep_mfma)
+            return seterr(h, PAR_SGNS_EINVAL,
+                          "write-back STORE_IN, STORE and HOT apply to the matrix-core repulsion on bf16 tables");
+        if (a->write_back == PAR_SGNS_WB_HOT && (a->hot_rows < 1 || a->hot_rows > h->cfg.vocab_size))
+            return seterr(h, PAR_SGNS_EINVAL, "write_back HOT needs 1 <= hot_rows <= vocab_size");
+        f.write_back = a->write_back;
+        f.hot_bytes = a->write_back == PAR_SGNS_WB_HOT ? (uint32_t)a->hot_rows * (uint32_t)h->ld * 2u : 0u;
         f.lds_per_wave = sgns::fast_lds_per_wave(h->R, h->nq, h->bf16, h->kmax, h->cfg.batch_size, h->ld, f.rep_mfma);
         f.lds_negs = sgns::fast_lds_negs(h->nq, h->bf16, h->cfg.batch_size, h->ld, f.rep_mfma);
         f.persistent = pers ? 1 : 0; f.rounding = h->cfg.rounding; f.compute_loss = a->compute_loss; f.iter = a->iter; f.epoch = a->epoch;
@@ -652,7 +663,7 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         if (pers && !planned && (rc = ensure_order(h, a->sent_begin, a->sent_end))) return rc;
         if (pers && a->use_graph) {
             GraphKey k{a->sent_begin, a->sent_end, a->epoch, a->compute_loss | (planned ? 2 : 0) | (f.rep_mfma ? 4 : 0), wpb, blocks, a->iter,
-                       f.grab, 0, a->seed, a->train_words};
+                       f.grab, 0, a->seed, a->train_words, f.hot_bytes | (uint32_t)f.write_back};
             std::memcpy(&k.alpha_bits, &a->alpha, 4);
             auto it = h->graphs.find(k);
             if (it == h->graphs.end()) {

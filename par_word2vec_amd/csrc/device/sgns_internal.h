@@ -6,6 +6,9 @@
 
 namespace sgns {
 
+// Write-back policies of the matrix-core instance (par_sgns_write_back).
+enum WbPolicy { kWbPolicyAtomic = 0, kWbPolicyStoreOut = 1, kWbPolicyStoreIn = 2, kWbPolicyStore = 3, kWbPolicyHot = 4 };
+
 // Device counters, one slot each (uint64), reset before every training call.
 enum Counter { kCntTokens = 0, kCntSentences, kCntPairs, kCntBatches, kCntNegatives, kCntRepPairs, kCntSlots = 8 };
 
@@ -42,6 +45,8 @@ struct FastArgs {
     int32_t grab;                 // persistent: queue slots (sentences) a wave takes per dequeue, >= 1
     int32_t rounding;             // bf16 write-back: 0 stochastic, 1 nearest
     int32_t rep_mfma;             // bf16 tables: repulsion products on the matrix cores
+    int32_t write_back;           // matrix-core instance: WbPolicy
+    uint32_t hot_bytes;           // kWbPolicyHot: rows at byte offsets >= this take plain stores
     int32_t compute_loss;
     int32_t iter, epoch;
     float a0;
@@ -70,6 +75,13 @@ struct WindowArgs {
 
 hipError_t launch_det(const DetArgs &a, hipStream_t st);
 hipError_t launch_fast(const FastArgs &a, int bf16, int np, int waves_per_block, int blocks, hipStream_t st);
+// Instance groups of the training kernel, one translation unit each (fast_inst_*.hip).
+hipError_t fast_vector_fp32(const FastArgs &a, int store_out, int np, int wpb, int blocks, hipStream_t st);
+hipError_t fast_vector_bf16(const FastArgs &a, int store_out, int np, int wpb, int blocks, hipStream_t st);
+hipError_t fast_mfma_atomic(const FastArgs &a, int np, int wpb, int blocks, hipStream_t st);
+hipError_t fast_mfma_store_out(const FastArgs &a, int np, int wpb, int blocks, hipStream_t st);
+hipError_t fast_mfma_store(const FastArgs &a, int store_in_only, int np, int wpb, int blocks, hipStream_t st);
+hipError_t fast_mfma_hot(const FastArgs &a, int np, int wpb, int blocks, hipStream_t st);
 hipError_t launch_reset(unsigned long long *counters, int n_counters, double *loss, unsigned long long *queue,
                         hipStream_t st);
 hipError_t launch_sample(const uint2 *alias, int V, uint64_t seed, int64_t sent, int64_t n, int32_t *out,

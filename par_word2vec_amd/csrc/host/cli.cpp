@@ -33,6 +33,8 @@ struct Opts {
     long long chunk_words = 10000000;
     int autotune = 0;  // persistent mode: tune the launch shape on the first chunk, this many runs per shape
     int min_sent_wave = 0;  // persistent mode: staleness floor of the default grid (0 = 16)
+    std::string write_back = "atomic";  // bf16 matrix-core write-back policy
+    int hot_rows = 0;                   // write_back hot: rows with id < this keep atomics
 };
 
 void usage() {
@@ -45,6 +47,7 @@ void usage() {
         "  -chunk-words <int> (10000000) -time-file <file> (PAR_Word2Vec_gpu_time)\n"
         "  -autotune <runs> (0): persistent mode, tune the launch shape on the first chunk first\n"
         "  -min-sentences-per-wave <int> (16): persistent mode, fewer waves on small corpora\n"
+        "  -write-back atomic|store_out|store_in|store|hot (atomic) -hot-rows <int>: bf16 row write-back\n"
         "  accepted and ignored: -threads -disk -gamma -max-num-sen\n");
 }
 
@@ -84,6 +87,8 @@ int main(int argc, char **argv) {
     num("-device", o.device);
     num("-autotune", o.autotune);
     num("-min-sentences-per-wave", o.min_sent_wave);
+    str("-write-back", o.write_back);
+    num("-hot-rows", o.hot_rows);
     str("-time-file", o.time_file);
     if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = std::strtoull(argv[i + 1], nullptr, 10);
     if ((i = arg_pos("-chunk-words", argc, argv)) > 0 && i + 1 < argc) o.chunk_words = std::atoll(argv[i + 1]);
@@ -140,6 +145,14 @@ int main(int argc, char **argv) {
     ta.seed = o.seed;
     ta.use_graph = mode == PAR_SGNS_PERSISTENT;
     ta.min_sentences_per_wave = o.min_sent_wave;
+    if (mode != PAR_SGNS_DETERMINISTIC) {
+        const char *wb[] = {"atomic", "store_out", "store_in", "store", "hot"};
+        ta.write_back = -1;
+        for (int k = 0; k < 5; k++)
+            if (o.write_back == wb[k]) ta.write_back = k;
+        if (ta.write_back < 0) return fail("-write-back", "atomic, store_out, store_in, store or hot");
+        ta.hot_rows = o.hot_rows;
+    }
     par_sgns_stats st{};
     if (mode == PAR_SGNS_DETERMINISTIC) {
         ph_plan *pl = ph_plan_build(stream.data(), n, counts.data(), V, T, o.window, o.negative, o.batch, o.iter,

@@ -17,6 +17,9 @@ WIN_MFMA, WIN_FP64 = 1, 2
 ROUNDINGS = {"stochastic": 0, "nearest": 1}
 # repulsion batch evaluation: matrix cores on bf16 tables unless "vector" is asked for
 REPULSIONS = {"auto": 0, "vector": 1, "matrix": 2}
+# bf16 matrix-core write-back (par_sgns_write_back): atomics, plain stores of W_out rows, of W_in
+# rows, of both tables, or of rows at or past hot_rows
+WRITE_BACKS = {"atomic": 0, "store_out": 1, "store_in": 2, "store": 3, "hot": 4}
 
 
 class DeviceError(RuntimeError):
@@ -111,11 +114,12 @@ class SgnsDevice:
               iters: int = 1, epoch: int = 0, seed: int = 1, compute_loss: bool = False,
               waves_per_block: int = 0, blocks: int = 0, use_graph: bool = False, planned: bool = False,
               prepare_only: bool = False, repulsion: str = "auto", sentences_per_dequeue: int = 0,
-              min_sentences_per_wave: int = 0) -> dict:
+              min_sentences_per_wave: int = 0, write_back: str = "atomic", hot_rows: int = 0) -> dict:
         m = MODES[mode] if isinstance(mode, str) else int(mode)
         args = N.ParSgnsTrainArgs(m, iters, epoch, int(compute_loss), alpha, waves_per_block, blocks,
                                   int(use_graph), int(planned), REPULSIONS[repulsion], train_words, seed,
-                                  sent_begin, sent_end, sentences_per_dequeue, min_sentences_per_wave)
+                                  sent_begin, sent_end, sentences_per_dequeue, min_sentences_per_wave,
+                                  WRITE_BACKS[write_back], hot_rows)
         if prepare_only:
             self._chk(self.lib.par_sgns_prepare(self._h, C.byref(args)), "prepare")
             return {}
@@ -125,12 +129,13 @@ class SgnsDevice:
 
     def autotune(self, sent_begin: int, sent_end: int, alpha: float, train_words: int, iters: int = 1,
                  epoch: int = 0, seed: int = 1, reps: int = 3, repulsion: str = "auto",
-                 min_sentences_per_wave: int = 0) -> dict:
+                 min_sentences_per_wave: int = 0, write_back: str = "atomic", hot_rows: int = 0) -> dict:
         """Times the persistent kernel's launch shapes on the sentence range
         (par_sgns_autotune; the tables are left as they were) and makes the
         fastest the handle's default.  Returns {"best": point, "points": [...]}."""
         args = N.ParSgnsTrainArgs(MODES["persistent"], iters, epoch, 0, alpha, 0, 0, 0, 0, REPULSIONS[repulsion],
-                                  train_words, seed, sent_begin, sent_end, 0, min_sentences_per_wave)
+                                  train_words, seed, sent_begin, sent_end, 0, min_sentences_per_wave,
+                                  WRITE_BACKS[write_back], hot_rows)
         cap = 64
         pts = (N.ParSgnsTunePoint * cap)()
         n = C.c_int32(0)
