@@ -8,12 +8,14 @@ one launch of the training kernel over the sentences of that chunk (a
 captured hipGraph for the persistent variant), inputs resident in HBM.
 
 Both variants train the same epoch from the same InitNet weights:
-  fp32 plain      one wavefront per sentence, fp32 tables, float atomics
+  fp32 plain      one wavefront per sentence, fp32 tables
   bf16 persistent work-queue kernel, bf16 tables, fp32 math, stochastic
                   rounding, repulsion products on the matrix cores, hipGraph
                   per chunk, launched with the saved launch-shape tuning of the
                   workload (par_word2vec_amd/tuned/; --autotune tunes here,
                   before the warmup)
+and both write W_in rows back with atomics and W_out rows with plain stores
+(--write-back; the reference CPU trainer updates rows in place, cpu:561-565).
 `value` is the bf16 persistent rate in Mwords/s: raw corpus words (every
 token incl. those removed by subsampling and "</s>") per second of the timed
 steps.  Each variant's tables are scored after the full epoch: planted-cluster
@@ -58,7 +60,7 @@ def algorithmic_bytes(tokens: int, negatives: int, dim: int, elem: int) -> float
 
 def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int, steps: int, csr, seed: int,
                 profile_only: bool = False, repulsion: str = "auto", workload: str = "1b",
-                tune: bool = False) -> dict:
+                tune: bool = False, write_back: str = "atomic") -> dict:
     w = c.workload
     V = len(c.vocab)
     dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision)
@@ -77,19 +79,20 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
         # process), or, with --autotune, the tuner run here on chunk 0
         if tune:
             b0, e0 = chunks[0]
-            res = dev.autotune(b0, e0, w.alpha, c.train_words, iters=w.iters, seed=seed, repulsion=repulsion)
+            res = dev.autotune(b0, e0, w.alpha, c.train_words, iters=w.iters, seed=seed, repulsion=repulsion,
+                               write_back=write_back)
             launch = dict(res["best"], source="autotuned in this run")
         else:
-            saved = autotune.load(workload, w, precision, repulsion)
+            saved = autotune.load(workload, w, precision, repulsion, write_back)
             if saved is not None:
-                launch = dict(autotune.apply(dev, saved),
-                              source=str(autotune.path_for(workload, precision, repulsion).relative_to(ROOT)))
+                launch = dict(autotune.apply(dev, saved), source=str(
+                    autotune.path_for(workload, precision, repulsion, write_back).relative_to(ROOT)))
         log(f"{mode}/{precision}: launch shape {launch or 'built-in default'}")
 
     def step(i: int, prepare_only: bool = False) -> dict:
         b, e = chunks[i % n]
         return dev.train(mode, b, e, w.alpha, c.train_words, iters=w.iters, epoch=0, seed=seed, use_graph=use_graph,
-                         prepare_only=prepare_only, repulsion=repulsion)
+                         prepare_only=prepare_only, repulsion=repulsion, write_back=write_back)
 
     for i in range(n):  # queue orders and graphs are built before anything is timed
         step(i, prepare_only=True)
@@ -119,7 +122,8 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     shape = stats[-1] if stats else {}
     # the same rate when the epoch's corpus upload (host buffers across PCIe) is charged pro rata
     pcie_s = wall + upload_s * raw / max(int(c.stream.size), 1)
-    out = dict(mode=mode, precision=precision, kernel=f"{precision}-{kind}", wall_s=wall, upload_s=upload_s,
+    out = dict(mode=mode, precision=precision, kernel=f"{precision}-{kind}", write_back=write_back, wall_s=wall,
+               upload_s=upload_s,
                pcie_inclusive_mwords_s=raw / pcie_s / 1e6, kernel_ms=kern_ms, raw_words=raw, tokens=tokens,
                negatives=negs, pairs=sum(s["pairs"] for s in stats), mwords_s=raw / wall / 1e6,
                kernel_mwords_s=raw / (kern_ms / 1e3) / 1e6, alg_bytes=byts, launches=len(stats),
@@ -213,6 +217,9 @@ def main() -> int:
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--chunk-words", type=int, default=10_000_000)
     ap.add_argument("--variants", default="fp32-plain,bf16-persistent")
+    ap.add_argument("--write-back", default="store_out", choices=["atomic", "store_out", "store_in", "store"],
+                    help="row write-back of both variants: W_in atomics and W_out plain stores (default), all "
+                         "atomics, or plain stores of W_in / both tables (bf16 matrix-core path only)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-score", action="store_true")
@@ -246,7 +253,9 @@ def main() -> int:
         prec, mode, *rep = v.split("-")  # precision-mode[-repulsion]
         results[v] = run_variant(c, mode, prec, chunks, raw_per_chunk, args.warmup, steps, csr, args.seed,
                                  profile_only=args.no_score, repulsion=rep[0] if rep else "auto",
-                                 workload=args.workload, tune=args.autotune)
+                                 workload=args.workload, tune=args.autotune,
+                                 write_back=args.write_back if (prec == "bf16" and not rep) or
+                                 args.write_back in ("atomic", "store_out") else "atomic")
     scores = {}
     if not args.no_score:
         windows = None
@@ -277,7 +286,8 @@ def main() -> int:
         "config": {"workload": w.name, "raw_words": int(c.stream.size), "vocab": len(c.vocab), "dim": w.dim,
                    "window": w.window, "negative": w.negative, "sample": w.sample, "batch_size": w.batch_size,
                    "chunk_words": args.chunk_words, "chunks": len(chunks), "trained_tokens": int(csr.tokens.size),
-                   "variant": f"{head['precision']} {head['mode']}", "parallelism": "single GPU"},
+                   "variant": f"{head['precision']} {head['mode']}", "write_back": head["write_back"],
+                   "parallelism": "single GPU"},
         "fp32_plain_mwords_s": round(base["mwords_s"], 2) if base else None,
         "speedup_vs_fp32_plain": round(head["mwords_s"] / base["mwords_s"], 3) if base else None,
         "quality": scores,

@@ -22,6 +22,11 @@ def test_load_matches_kernel_shape(tmp_path, monkeypatch):
     assert A.load("t", _w(dim=128), "bf16") is None        # another kernel instance
     assert A.load("t", w, "fp32") is None                  # no file for it
     assert A.load("other", w, "bf16") is None
+    assert A.load("t", w, "bf16", write_back="store_out") is None   # another write-back instance
+    A.path_for("t", "bf16", write_back="store_out").write_text(
+        json.dumps({"key": A.shape_key(w, "bf16", write_back="store_out"), "best": best, "points": [best]}))
+    assert A.load("t", w, "bf16", write_back="store_out")["best"] == best
+    assert A.path_for("t", "bf16", write_back="store_out") != A.path_for("t", "bf16")
 
 
 def test_apply_passes_the_best_shape():
@@ -40,8 +45,9 @@ def test_saved_tunings_match_their_workloads():
     (bench.py then launches with them instead of the built-in default)."""
     from par_word2vec_amd.corpora import WORKLOADS
     for name in ("1b", "text8", "sbm"):
-        t = A.load(name, WORKLOADS[name], "bf16")
-        assert t is not None, name
-        b = t["best"]
-        assert 1 <= b["waves_per_block"] <= 8 and b["waves_per_cu"] % b["waves_per_block"] == 0
-        assert b in t["points"]
+        for wb in ("atomic", "store_out"):
+            t = A.load(name, WORKLOADS[name], "bf16", write_back=wb)
+            assert t is not None, (name, wb)
+            b = t["best"]
+            assert 1 <= b["waves_per_block"] <= 8 and b["waves_per_cu"] % b["waves_per_block"] == 0
+            assert b in t["points"]

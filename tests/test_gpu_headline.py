@@ -3,7 +3,8 @@
 window 5, 10 negatives, sample 1e-5), one epoch in ~10M-raw-word chunks, with
 the benchmark's two variants: bf16 tables on the persistent kernel (matrix-core
 repulsion, stochastic rounding, one hipGraph per chunk, the saved launch shape)
-and fp32 tables on the plain kernel.
+and fp32 tables on the plain kernel, both with the benchmark's write-back (W_in
+rows by atomics, W_out rows by plain stores).
 
 Bars (the metric's "at equal planted-cluster score", with scores that do not
 saturate):
@@ -41,6 +42,9 @@ def onebillion():
     return c, csr, windows
 
 
+WRITE_BACK = "store_out"  # bench.py's default
+
+
 def _epoch(c, csr, mode, precision):
     w = c.workload
     with SgnsDevice(len(c.vocab), w.dim, w.window, w.negative, w.batch_size, precision) as dev:
@@ -49,13 +53,13 @@ def _epoch(c, csr, mode, precision):
         dev.set_sampler(H.unigram_entries(c.counts))
         dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
         if mode == "persistent":
-            saved = autotune.load("1b", w, precision, "auto")
+            saved = autotune.load("1b", w, precision, "auto", WRITE_BACK)
             if saved is not None:
                 autotune.apply(dev, saved)
         tot = dict(tokens=0, sentences=0, pairs=0, rep_pairs=0, loss_attr=0.0, loss_rep=0.0, kernel_ms=0.0)
         for b, e in chunk_bounds(csr.sent_raw, 10_000_000):
             st = dev.train(mode, b, e, w.alpha, c.train_words, seed=1, compute_loss=True,
-                           use_graph=mode == "persistent")
+                           use_graph=mode == "persistent", write_back=WRITE_BACK)
             for k in tot:
                 tot[k] += st[k]
         w_in, w_out = dev.get_weights()
