@@ -20,7 +20,8 @@ i32, i64, u64, f32, f64, vp, cp = C.c_int32, C.c_int64, C.c_uint64, C.c_float, C
 
 class Params(C.Structure):
     _fields_ = [("dim", i32), ("window", i32), ("negative", i32), ("batch_size", i32), ("iter", i32),
-                ("threads", i32), ("alpha", f32), ("sample", f32)]
+                ("threads", i32), ("alpha", f32), ("sample", f32), ("seed", u64), ("bf16", i32), ("strong_rng", i32),
+                ("alpha_per_sentence", i32), ("reserved", i32)]
 
 
 class Stats(C.Structure):
@@ -39,7 +40,8 @@ class Trace(C.Structure):
 
 
 class Curve(C.Structure):
-    _fields_ = [("bucket_words", i64), ("n_buckets", i64), ("loss", vp), ("pairs", vp)]
+    _fields_ = [("bucket_words", i64), ("n_buckets", i64), ("loss", vp), ("pairs", vp), ("loss_attr", vp),
+                ("pairs_attr", vp)]
 
 
 def build() -> None:
@@ -191,13 +193,19 @@ class TrainResult:
     stats: dict
     trace: dict | None = None
     curve: tuple[np.ndarray, np.ndarray] | None = None
+    curve_attr: tuple[np.ndarray, np.ndarray] | None = None  # attraction pairs only
 
 
 def train(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, window: int, negative: int,
           batch_size: int, iters: int, alpha: float, sample: float, w_in: np.ndarray | None = None,
           w_out: np.ndarray | None = None, table: np.ndarray | None = None, want_trace: bool = False,
-          bucket_words: int = 0, n_buckets: int = 0) -> TrainResult:
-    """Single-thread deterministic run (reference with -threads 1)."""
+          bucket_words: int = 0, n_buckets: int = 0, seed: int = 0, bf16: bool = False,
+          strong_rng: bool = False, alpha_per_sentence: bool = False) -> TrainResult:
+    """Single-thread deterministic run (reference with -threads 1).  seed and
+    bf16 are test-only options: another LCG seed than thread 0's, and bf16
+    tables with stochastic rounding at the GPU trainer's write-back points;
+    strong_rng replaces the reference's LCG by a well-mixed generator;
+    alpha_per_sentence sets each sentence's rate from the words before it."""
     L = lib()
     counts = np.ascontiguousarray(counts, np.uint32)
     V = counts.size
@@ -211,7 +219,12 @@ def train(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, wi
     if table is None:
         table = unigram_table(counts)
     et = exp_table()
-    p = Params(dim, window, negative, batch_size, iters, 1, alpha, sample)
+    p = Params(dim, window, negative, batch_size, iters, 1, alpha, sample, seed, int(bf16), int(strong_rng),
+               int(alpha_per_sentence), 0)
+    if bf16:  # tables start as the bf16 values the GPU's upload rounds them to (nearest even)
+        for a in (w_in, w_out):
+            b = a.view(np.uint32)
+            b[...] = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
     st = Stats()
     tr = None
     keep = []
@@ -229,7 +242,8 @@ def train(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, wi
     cv = None
     if n_buckets:
         cl, cpairs = np.zeros(n_buckets), np.zeros(n_buckets)
-        cv = Curve(bucket_words, n_buckets, P(cl), P(cpairs))
+        al, apairs = np.zeros(n_buckets), np.zeros(n_buckets)
+        cv = Curve(bucket_words, n_buckets, P(cl), P(cpairs), P(al), P(apairs))
     rc = L.orc_train(C.byref(p), P(s), stream.size, P(counts), V, train_words, P(table), P(et), P(w_in),
                      P(w_out), C.byref(st), C.byref(tr) if tr is not None else None,
                      C.byref(cv) if cv is not None else None)
@@ -244,7 +258,8 @@ def train(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, wi
                      sent_len=a["sent_len"][: tr.n_sent].copy(), sent_alpha=a["sent_alpha"][: 2 * tr.n_sent].copy(),
                      rep_b=a["rep_b"][: tr.n_batches].copy(), negs=a["negs"][: tr.n_negs].copy())
     curve = (cl, cpairs) if n_buckets else None
-    return TrainResult(w_in, w_out, st.as_dict(), trace, curve)
+    curve_attr = (al, apairs) if n_buckets else None
+    return TrainResult(w_in, w_out, st.as_dict(), trace, curve, curve_attr)
 
 
 def train_hogwild(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, window: int, negative: int,
@@ -259,7 +274,7 @@ def train_hogwild(stream: np.ndarray, counts: np.ndarray, train_words: int, dim:
     if table is None:
         table = unigram_table(counts)
     et = exp_table()
-    p = Params(dim, window, negative, batch_size, iters, threads, alpha, sample)
+    p = Params(dim, window, negative, batch_size, iters, threads, alpha, sample, 0, 0, 0, 0, 0)
     st = Stats()
     secs = L.orc_train_hogwild(C.byref(p), P(s), s.size, P(counts), V, train_words, P(table), P(et), P(w_in),
                                P(w_out), C.byref(st))

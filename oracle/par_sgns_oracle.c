@@ -416,6 +416,33 @@ static inline void trace_sentence(orc_trace *tr, const int32_t *sen, int L, floa
     tr->n_sent++;
 }
 
+/* Test-only replacement of the LCG (orc_params.strong_rng): the SplitMix64
+ * finalizer (Steele, Lea, Flood, "Fast splittable pseudorandom number
+ * generators", OOPSLA 2014) of x + the golden-ratio increment -- every bit of
+ * a draw is well mixed, so x % 2W has no parity pattern. */
+static inline uint64_t strong_step(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* bf16 emulation (orc_params.bf16): stochastic rounding of a row to bf16
+ * (16 uniform bits added below the kept mantissa, then truncation). */
+static void round_row_bf16(float *row, int d, uint32_t *rs) {
+    for (int j = 0; j < d; j++) {
+        uint32_t x = *rs;
+        x ^= x << 13;
+        x ^= x >> 17;
+        x ^= x << 5;
+        *rs = x;
+        uint32_t b;
+        memcpy(&b, &row[j], 4);
+        b = (b + (x & 0xFFFFu)) & 0xFFFF0000u;
+        memcpy(&row[j], &b, 4);
+    }
+}
+
 /* The body of the per-thread loop at cpu:397-700.  With one thread and the
  * thread-0 seed this is the reference's deterministic schedule. */
 static void orc_thread_run(orc_thread *t) {
@@ -440,6 +467,8 @@ static void orc_thread_run(orc_thread *t) {
     const float *et = t->et;
     float *w_in = t->w_in, *w_out = t->w_out;
     const uint64_t denom_words = (uint64_t)p->iter * t->T + 1;
+    uint32_t rs = (uint32_t)(p->seed * 2654435761u) ^ 0x9E3779B9u;  /* bf16 emulation noise */
+    if (!rs) rs = 1;
 
     for (;;) {
         /* cpu:431-446 learning-rate update, checked before loading */
@@ -461,6 +490,10 @@ static void orc_thread_run(orc_thread *t) {
         /* cpu:447-471 sentence build with subsampling */
         if (sentence_length == 0) {
             sent_key = epoch_base + word_count;
+            if (p->alpha_per_sentence) { /* test-only: the rate of the words before this sentence */
+                alpha = starting_alpha * (1 - (float)(epoch_base + word_count) / (float)denom_words);
+                if (alpha < starting_alpha * 0.0001f) alpha = starting_alpha * 0.0001f;
+            }
             for (;;) {
                 int w = t->stream[word_count];
                 word_count++;
@@ -468,7 +501,7 @@ static void orc_thread_run(orc_thread *t) {
                 if (p->sample > 0) {
                     float ratio = (p->sample * (float)t->T) / (float)t->cn[w];
                     float ran = sqrtf(ratio) + ratio;
-                    x = orc_lcg(x);
+                    x = p->strong_rng ? strong_step(x) : orc_lcg(x);
                     t->st.draws++;
                     if (MUT(ORC_MUT_SUBSAMPLE_FLIP) ? !(ran < (float)(x & 0xFFFF) / 65536.f)
                                                     : ran < (float)(x & 0xFFFF) / 65536.f)
@@ -517,7 +550,7 @@ static void orc_thread_run(orc_thread *t) {
 
         /* cpu:529-569 attraction for one centre */
         int target = sen[sentence_position < ORC_MAX_SENTENCE ? sentence_position : 0];
-        x = orc_lcg(x);
+        x = p->strong_rng ? strong_step(x) : orc_lcg(x);
         t->st.draws++;
         int b = (int)(x % (uint64_t)W);
         if (sentence_length > 0) {
@@ -548,10 +581,31 @@ static void orc_thread_run(orc_thread *t) {
             t->st.loss_attr += lp;
             if (t->cv && t->cv->n_buckets > 0) {
                 int64_t k = (int64_t)(sent_key / (uint64_t)t->cv->bucket_words);
-                if (k < t->cv->n_buckets) { t->cv->loss[k] += lp; t->cv->pairs[k] += 1; }
+                if (k < t->cv->n_buckets) {
+                    t->cv->loss[k] += lp;
+                    t->cv->pairs[k] += 1;
+                    if (t->cv->loss_attr) { t->cv->loss_attr[k] += lp; t->cv->pairs_attr[k] += 1; }
+                }
             }
         }
         sentence_position++;
+        if (p->bf16 && sentence_length > 0) {
+            /* the GPU trainer's write-back points: the centre's W_out row after its
+             * pairs (kept in registers while the next centre is the same word), and
+             * the W_in row of the position leaving the window unless its word is
+             * still referenced by the window of the next centre */
+            const int pc = sentence_position - 1;
+            if (!(sentence_position < sentence_length && sen[sentence_position] == target))
+                round_row_bf16(w_out + (int64_t)target * d, d, &rs);
+            const int r = pc - W;
+            if (r >= 0) {
+                int live = 0;
+                for (int q = r + 1; q <= pc + 1 + W && q < sentence_length; q++) live |= sen[q] == sen[r];
+                if (!live) round_row_bf16(w_in + (int64_t)sen[r] * d, d, &rs);
+            }
+            if (sentence_position >= sentence_length) /* flush: rounding a bf16 value changes nothing */
+                for (int q = 0; q < sentence_length; q++) round_row_bf16(w_in + (int64_t)sen[q] * d, d, &rs);
+        }
 
         /* cpu:571-679 repulsion over all batches once the sentence is done */
         if (sentence_position >= sentence_length) {
@@ -564,7 +618,7 @@ static void orc_thread_run(orc_thread *t) {
                     int input_size = max_pos - min_pos + 1;
                     if (input_size <= 0) break;
                     int rep_b;
-                    x = orc_lcg(x);
+                    x = p->strong_rng ? strong_step(x) : orc_lcg(x);
                     t->st.draws++;
                     const int end_gap = MUT(ORC_MUT_BATCH_END) ? sentence_length - min_pos : (sentence_length - 1) - min_pos;
                     if (MUT(ORC_MUT_REPB_2W)) rep_b = (int)(x % (uint64_t)(W * 2));
@@ -573,7 +627,7 @@ static void orc_thread_run(orc_thread *t) {
                     else rep_b = (int)(x % (uint64_t)(W * 2));
                     int K = rep_b * NEG;
                     for (int ng = 0; ng < K; ng++) {
-                        x = orc_lcg(x);
+                        x = p->strong_rng ? strong_step(x) : orc_lcg(x);
                         t->st.draws++;
                         int s = t->table[(MUT(ORC_MUT_TABLE_INDEX) ? x : x >> 16) % (uint64_t)ORC_TABLE_SIZE];
                         if (!s) s = (int)(x % (uint64_t)(t->V - 1) + 1);
@@ -583,7 +637,7 @@ static void orc_thread_run(orc_thread *t) {
                             for (int q = 0; q < sentence_length; q++)
                                 if (sen[q] == s) { found = 1; break; }
                             if (!found) break;
-                            x = orc_lcg(x);
+                            x = p->strong_rng ? strong_step(x) : orc_lcg(x);
                             t->st.draws++;
                             s = (int)(x % (uint64_t)(t->V - 1) + 1);
                         }
@@ -632,6 +686,10 @@ static void orc_thread_run(orc_thread *t) {
                         int64_t src = (int64_t)i * d, des = (int64_t)shared_negative[i] * d;
                         for (int j = 0; j < d; j++) w_out[des + j] += M_out_upd[src + j];
                     }
+                    if (p->bf16) {
+                        for (int i = 0; i < input_size; i++) round_row_bf16(w_in + (int64_t)sen[min_pos + i] * d, d, &rs);
+                        for (int i = 0; i < K; i++) round_row_bf16(w_out + (int64_t)shared_negative[i] * d, d, &rs);
+                    }
                     rep_sentence_position += input_size;
                 }
             }
@@ -675,7 +733,7 @@ int orc_train(const orc_params *p, const int32_t *stream, uint64_t local_words,
     t.et = exp_table;
     t.w_in = w_in;
     t.w_out = w_out;
-    t.rng = 0; /* thread id 0, cpu:401 */
+    t.rng = p->seed; /* thread id 0 -> 0, cpu:401 */
     t.tr = trace;
     t.cv = curve;
     t.wc_actual = &wca;

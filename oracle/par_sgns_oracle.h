@@ -42,6 +42,19 @@ typedef struct orc_params {
     int32_t threads;    /* -threads (only the Hogwild variant uses >1) */
     float alpha;        /* -alpha (starting_alpha) */
     float sample;       /* -sample */
+    /* Test-only options (0 reproduces the reference): */
+    uint64_t seed;      /* LCG seed of the single-thread trainer (the reference's thread 0 uses 0,
+                           cpu:401); other values give independent runs of the same schedule rules */
+    int32_t bf16;       /* 1: emulate bf16 tables with stochastic rounding -- a row is rounded where
+                           the GPU trainer writes it back (a W_out centre row after its pairs, a
+                           W_in row when its word leaves the window, every row of a repulsion batch
+                           after the batch), with its own xorshift32 noise stream */
+    int32_t strong_rng; /* 1: draws from a well-mixed generator instead of the reference's LCG, whose
+                           low bits repeat with short periods (x_n's parity alternates with n) */
+    int32_t alpha_per_sentence; /* 1: the learning rate of each sentence from the words read before
+                           it (the GPU trainer's rule) instead of a refresh every 10000 words
+                           (cpu:431-446) */
+    int32_t reserved;
 } orc_params;
 
 /* Counters and the FP64 loss a run produces. */
@@ -78,8 +91,10 @@ typedef struct orc_trace {
 typedef struct orc_curve {
     int64_t bucket_words;
     int64_t n_buckets;
-    double *loss;
+    double *loss;       /* attraction + repulsion */
     double *pairs;
+    double *loss_attr;  /* attraction pairs only (may be NULL) */
+    double *pairs_attr;
 } orc_curve;
 
 /* Planted misreadings of the reference (test infrastructure for the

@@ -8,10 +8,47 @@ Bars:
                                schedule: tables within 1e-4 of the update scale,
                                losses within 1e-6 relative (only the order of
                                float additions differs)
-  Hogwild kernel, Philox       its own random streams: loss per pair within 1%
-                               of the oracle's over the epoch and per tenth of
-                               the corpus within 3%, planted-cluster recall@10
-                               within 0.015 of the oracle's
+  Hogwild kernel, Philox       its own random streams against the oracle's
+                               LCG streams, each side averaged over 8 seeds
+                               (one stream against another differs by up to
+                               ~1.1 % per tenth of the corpus even between two
+                               oracle seeds): loss per attraction pair and per
+                               repulsion pair of every tenth within 1 %, of the
+                               epoch within 0.5 %, and planted-cluster
+                               recall@10 within 1 % (fp32) -- per tenth, the
+                               bar is 1 % or 3 standard errors of the
+                               difference of the two means (from the seed
+                               spread of each side), whichever is larger: in
+                               the tenths where the loss takes off one seed of
+                               the oracle alone moves it by ~1.5 %; bf16 tables with
+                               stochastic rounding against the oracle with the
+                               same rounding at the same write-back points:
+                               the same loss bars.  For these bars the oracle
+                               runs the reference's rules with the kernel's two
+                               deliberate substitutions, each measured against
+                               the reference on the CPU: (1) a well-mixed
+                               generator in place of the LCG, whose low bits
+                               repeat with short periods (x_n's parity
+                               alternates with n, so with an odd -negative
+                               x % 2W favours odd rep_b; its low 16 bits, which
+                               decide subsampling, repeat every 65,536 draws) --
+                               the epoch losses move by -0.4 % / +0.5 %
+                               (attraction / repulsion); (2) each sentence's
+                               learning rate from the words before it, where
+                               the reference refreshes it every 10,000 words
+                               (cpu:431-446) -- up to -0.5 % / +0.5 % per tenth.
+                               Against the reference rules themselves (LCG,
+                               10,000-word refresh) the epoch losses must agree
+                               within 1.5 %.  The two pair kinds are compared
+                               apart, as the number of negatives drawn shifts
+                               their proportions.  Concurrency: one wavefront
+                               meets these bars with room (every tenth within
+                               0.7 %); the default grid (~6 wavefronts on a
+                               100-sentence tenth) adds a Hogwild effect of up
+                               to 1.5 % per tenth (a faster take-off, -1.5 %
+                               attraction in tenth 4; +0.9 % / -1.0 % in the
+                               last tenth), so there the per-tenth floor is
+                               1.5 %; the epoch bars are the same.
 """
 import numpy as np
 import pytest
@@ -239,36 +276,89 @@ def test_matrix_repulsion_needs_bf16():
             dev.train("persistent", 0, plan.n_sent, 0.025, vocab.train_words, planned=True, repulsion="matrix")
 
 
-def test_hogwild_philox_loss_curve_and_recall(zipf1m):
-    """fp32 persistent kernel with its own Philox streams and on-device alias
-    sampling, trained chunk by chunk (a tenth of the corpus each, the default
-    grid): loss per pair of every tenth and of the epoch against the oracle's
-    loss curve, and planted-cluster recall."""
-    c, ref, bucket = zipf1m
+SEEDS = 8
+
+
+@pytest.fixture(scope="module")
+def oracle_curves(zipf1m):
+    """The oracle's loss per pair per tenth of the corpus, epoch loss and
+    recall, for seeds 0..3, fp32 and bf16-with-stochastic-rounding tables,
+    with well-mixed draws and with the reference's LCG."""
+    c, _, bucket = zipf1m
     w = c.workload
-    ref_recall = E.planted_cluster_recall(ref.w_in, c.groups)
-    ref_total = (ref.stats["loss_attr"] + ref.stats["loss_rep"]) / (ref.stats["pairs"] + ref.stats["rep_pairs"])
-    curve_loss, curve_pairs = ref.curve
-    csr = H.subsample(c.stream, c.counts, c.train_words, w.sample, seed=11)
-    chunks = chunk_bounds(csr.sent_raw, bucket)
-    assert len(chunks) == 10
-    with SgnsDevice(len(c.vocab), w.dim, w.window, w.negative, w.batch_size, "fp32") as dev:
-        dev.set_exp_table(H.exp_table())
-        dev.set_weights(*H.init_net(len(c.vocab), w.dim))
-        dev.set_sampler(H.unigram_entries(c.counts))
-        dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
-        per = [dev.train("persistent", b, e, w.alpha, c.train_words, compute_loss=True, seed=5) for b, e in chunks]
-        g_in, _ = dev.get_weights()
-    gl = np.array([(s["loss_attr"] + s["loss_rep"]) / (s["pairs"] + s["rep_pairs"]) for s in per])
-    rl = curve_loss / curve_pairs
-    total = sum(s["loss_attr"] + s["loss_rep"] for s in per) / sum(s["pairs"] + s["rep_pairs"] for s in per)
-    recall = E.planted_cluster_recall(g_in, c.groups)
-    print(f"loss/pair {total:.5f} vs {ref_total:.5f}; curve rel err {np.round(gl / rl - 1, 4)}; "
-          f"recall {recall:.4f} vs {ref_recall:.4f}", flush=True)
-    assert abs(total / ref_total - 1) < 0.01
-    assert np.all(np.abs(gl / rl - 1) < 0.03)
-    assert abs(recall - ref_recall) < 0.015
-    assert sum(s["tokens"] for s in per) == csr.tokens.size
+    out = {}
+    from concurrent.futures import ThreadPoolExecutor  # the oracle's C calls release the GIL
+
+    def run(k, bf16, strong):  # strong: the kernel's substitutions (well-mixed draws, rate per sentence)
+        return O.train(c.stream, c.counts, c.train_words, w.dim, w.window, w.negative, w.batch_size, 1, w.alpha,
+                       w.sample, bucket_words=bucket, n_buckets=10, seed=k, bf16=bf16, strong_rng=strong,
+                       alpha_per_sentence=strong)
+
+    with ThreadPoolExecutor(16) as ex:
+        futs = {(bf16, strong): [ex.submit(run, k, bf16, strong) for k in range(SEEDS)]
+                for bf16, strong in ((False, True), (True, True), (False, False), (True, False))}
+    for (bf16, strong), fs in futs.items():
+        runs = [f.result() for f in fs]
+        a = np.array([r.curve_attr[0] / r.curve_attr[1] for r in runs])
+        rp = np.array([(r.curve[0] - r.curve_attr[0]) / (r.curve[1] - r.curve_attr[1]) for r in runs])
+        out[bf16, strong] = dict(
+            attr=a.mean(0), rep=rp.mean(0), attr_var=a.var(0, ddof=1) / SEEDS, rep_var=rp.var(0, ddof=1) / SEEDS,
+            total_attr=np.mean([r.stats["loss_attr"] / r.stats["pairs"] for r in runs]),
+            total_rep=np.mean([r.stats["loss_rep"] / r.stats["rep_pairs"] for r in runs]),
+            recall=np.mean([E.planted_cluster_recall(r.w_in, c.groups) for r in runs]))
+    return out
+
+
+@pytest.mark.parametrize("grid", ["one wave", "default"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_hogwild_philox_loss_curve_and_recall(zipf1m, oracle_curves, precision, grid):
+    """Persistent kernel with its own Philox streams and on-device alias
+    sampling, trained chunk by chunk (a tenth of the corpus each, the default
+    grid), 8 seeds: the mean loss per pair of every tenth and of the epoch
+    against the mean of 8 oracle seeds, and planted-cluster recall (fp32; on
+    this workload bf16 storage washes the clusters out in the oracle too)."""
+    c, _, bucket = zipf1m
+    w = c.workload
+    ref = oracle_curves[precision == "bf16", True]
+    lcg = oracle_curves[precision == "bf16", False]
+    attr, rep, tot_a, tot_r, recalls = [], [], [], [], []
+    for k in range(SEEDS):
+        csr = H.subsample(c.stream, c.counts, c.train_words, w.sample, seed=11 + k)
+        chunks = chunk_bounds(csr.sent_raw, bucket)
+        assert len(chunks) == 10
+        with SgnsDevice(len(c.vocab), w.dim, w.window, w.negative, w.batch_size, precision) as dev:
+            dev.set_exp_table(H.exp_table())
+            dev.set_weights(*H.init_net(len(c.vocab), w.dim))
+            dev.set_sampler(H.unigram_entries(c.counts))
+            dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
+            shape = dict(blocks=1, waves_per_block=1) if grid == "one wave" else {}
+            per = [dev.train("persistent", b, e, w.alpha, c.train_words, compute_loss=True, seed=5 + k, **shape)
+                   for b, e in chunks]
+            g_in, _ = dev.get_weights()
+        assert sum(s["tokens"] for s in per) == csr.tokens.size
+        attr.append([s["loss_attr"] / s["pairs"] for s in per])
+        rep.append([s["loss_rep"] / s["rep_pairs"] for s in per])
+        tot_a.append(sum(s["loss_attr"] for s in per) / sum(s["pairs"] for s in per))
+        tot_r.append(sum(s["loss_rep"] for s in per) / sum(s["rep_pairs"] for s in per))
+        recalls.append(E.planted_cluster_recall(g_in, c.groups))
+    attr, rep = np.array(attr), np.array(rep)
+    ea, er = attr.mean(0) / ref["attr"] - 1, rep.mean(0) / ref["rep"] - 1
+    # 3 standard errors of the difference of the two seed means, relative
+    floor = 0.01 if grid == "one wave" else 0.015
+    ba = np.maximum(floor, 3 * np.sqrt(attr.var(0, ddof=1) / SEEDS + ref["attr_var"]) / ref["attr"])
+    br = np.maximum(floor, 3 * np.sqrt(rep.var(0, ddof=1) / SEEDS + ref["rep_var"]) / ref["rep"])
+    ta, tr = np.mean(tot_a) / ref["total_attr"] - 1, np.mean(tot_r) / ref["total_rep"] - 1
+    recall = np.mean(recalls)
+    la, lr = np.mean(tot_a) / lcg["total_attr"] - 1, np.mean(tot_r) / lcg["total_rep"] - 1
+    print(f"[philox {precision}, {grid}] epoch rel err attraction {ta:+.4f} repulsion {tr:+.4f} (LCG oracle {la:+.4f} "
+          f"{lr:+.4f}); per tenth attraction {np.round(ea, 4)} (bar {np.round(ba, 4)}) repulsion {np.round(er, 4)} "
+          f"(bar {np.round(br, 4)}); recall {recall:.4f} vs {ref['recall']:.4f} (LCG oracle {lcg['recall']:.4f})",
+          flush=True)
+    assert abs(ta) < 0.005 and abs(tr) < 0.005
+    assert abs(la) < 0.015 and abs(lr) < 0.015
+    assert np.all(np.abs(ea) < ba) and np.all(np.abs(er) < br)
+    if precision == "fp32":
+        assert abs(recall / ref["recall"] - 1) < 0.01
 
 
 def test_plain_mode_counts(zipf1m):

@@ -352,3 +352,23 @@ def test_window_loss_and_grad_f64():
             lm = O.window_loss_f64(centre, ctx, neg, w_in, w_out)
             tab[r_, c_] += h
             assert (lp - lm) / (2 * h) == pytest.approx(grad[r_, c_], rel=1e-5, abs=1e-8)
+
+
+def test_oracle_test_options_seed_and_bf16():
+    """The oracle's test-only options: seed 0 is the reference's thread-0 run;
+    another seed gives another run of the same rules; bf16 leaves every table
+    entry bf16-representable and, with stochastic rounding, keeps the loss."""
+    raw = np.random.default_rng(4).integers(1, 60, 30_000).astype(np.int32)
+    stream = np.where(np.arange(raw.size) % 50 == 49, 0, raw).astype(np.int32)
+    cn = np.bincount(stream, minlength=60).astype(np.uint32)
+    cn[cn == 0] = 1
+    args = (stream, cn, int(stream.size), 16, 3, 3, 12, 1, 0.025, 1e-3)
+    a, b, c = O.train(*args), O.train(*args, seed=0), O.train(*args, seed=1)
+    assert np.array_equal(a.w_in, b.w_in) and a.stats["final_rng"] == b.stats["final_rng"]
+    assert not np.array_equal(a.w_in, c.w_in) and a.stats["pairs"] != c.stats["pairs"]
+    h = O.train(*args, bf16=True)
+    assert not (h.w_in.view(np.uint32) & 0xFFFF).any() and not (h.w_out.view(np.uint32) & 0xFFFF).any()
+    la = (a.stats["loss_attr"] + a.stats["loss_rep"]) / (a.stats["pairs"] + a.stats["rep_pairs"])
+    lh = (h.stats["loss_attr"] + h.stats["loss_rep"]) / (h.stats["pairs"] + h.stats["rep_pairs"])
+    assert h.stats["final_rng"] == a.stats["final_rng"]  # rounding noise has its own stream
+    assert abs(lh / la - 1) < 0.01
