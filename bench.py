@@ -24,12 +24,16 @@ SGNS loss per pair of 65,536 held-out windows (Zipf workloads), or community
 kNN accuracy (SBM walks).
 
 The CPU baseline is the oracle's OpenMP Hogwild restatement of the
-reference CPU trainer, timed on a bounded prefix of the same corpus."""
+reference CPU trainer, run as the reference's harness runs it (text file,
+per-thread byte-offset slices, loading inside the timed region) on a bounded
+prefix of the same corpus, with the host's CPU share of threads (lscpu's
+model, sockets and cores are reported beside it)."""
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -184,28 +188,81 @@ def quality(c, w_in, w_out, windows) -> dict:
     return out
 
 
-def cpu_baseline(c, csr, seconds: float) -> dict:
+def host_cpu() -> dict:
+    """lscpu's model / sockets / cores / threads of this host, and the CPUs
+    this process may run on."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        info = {k.strip(): v.strip() for k, _, v in (ln.partition(":") for ln in out.splitlines())}
+    except (OSError, subprocess.SubprocessError) as ex:
+        log(f"lscpu failed: {ex!r}")
+
+    def num(key):
+        try:
+            return int(info.get(key, ""))
+        except ValueError:
+            return None
+    return {"model": info.get("Model name"), "sockets": num("Socket(s)"), "cores_per_socket": num("Core(s) per socket"),
+            "threads_per_core": num("Thread(s) per core"), "online_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_threads() -> int:
+    """The CPU share of this process: OMP_NUM_THREADS where the host sets it
+    (a GPU's share of a shared host), else every CPU it may run on."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(c, seconds: float) -> dict:
+    """The oracle's Hogwild restatement of PAR_Word2Vec_cpu.cpp run the way
+    the reference's harness runs it (cpu:395-416, 702-706): the sample is
+    written as a text file and a "word count" vocabulary file, ReadVocab
+    reads the vocabulary, each thread seeks to its byte offset, loads its
+    share of words and trains it, and the time covers loading and training.
+    The vocabulary keeps the whole corpus's counts, so subsampling and the
+    learning-rate schedule are the full corpus's."""
+    import shutil
+    import tempfile
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
     w = c.workload
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     # bounded sample: a prefix of the corpus sized to roughly `seconds` of CPU
-    # work, calibrated by a short probe run
+    # work, calibrated by a short in-memory probe run
     table = O.unigram_table(c.counts)
     probe = min(c.stream.size, 400_000)
     cut = int(np.flatnonzero(c.stream[:probe] == 0)[-1]) + 1
     secs, _, _ = O.train_hogwild(c.stream[:cut], c.counts, c.train_words, w.dim, w.window, w.negative,
                                  w.batch_size, 1, w.alpha, w.sample, threads, table=table)
+    del table
     rate = cut / max(secs, 1e-6)
     n = int(min(c.stream.size, max(cut, rate * seconds)))
     ends = np.flatnonzero(c.stream[:n] == 0)
     n = int(ends[-1]) + 1 if ends.size else n
-    secs, st, _ = O.train_hogwild(c.stream[:n], c.counts, c.train_words, w.dim, w.window, w.negative, w.batch_size,
-                                  1, w.alpha, w.sample, threads, table=table)
+    d = tempfile.mkdtemp(prefix="sgns_cpu_")
+    try:
+        words = [c.vocab.word(i) for i in range(len(c.vocab))]
+        text, vpath = os.path.join(d, "sample.txt"), os.path.join(d, "vocab.txt")
+        t0 = time.perf_counter()
+        size = O.write_text_corpus(text, c.stream[:n], words)
+        O.write_vocab(vpath, words, c.counts)
+        log(f"cpu baseline: {n} words, {size / 1e6:.0f} MB of text written in {time.perf_counter() - t0:.1f} s; "
+            f"{threads} threads")
+        v = O.OracleVocab.read(vpath, text, w.min_count, flavour="fast")
+        r = O.train_hogwild_file(v, text, n, v.train_words, w.dim, w.window, w.negative, w.batch_size, 1,
+                                 w.alpha, w.sample, threads)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    st, secs = r["stats"], r["seconds"]
     return {"value": round(n / secs / 1e6, 3), "unit": "Mwords/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} raw words ({st['trained_tokens']} trained tokens) of the same corpus, "
-                      f"1 epoch, {secs:.1f} s, OpenMP Hogwild restatement of PAR_Word2Vec_cpu.cpp"}
+            "sample": f"first {n} raw words ({st['trained_tokens']} trained tokens) of the same corpus as text, "
+                      f"1 epoch, {secs:.1f} s including the per-thread file-offset loads (longest "
+                      f"{r['load_max']:.1f} s); OpenMP Hogwild restatement of PAR_Word2Vec_cpu.cpp with naive "
+                      f"GEMM loops in place of MKL",
+            "load_max_s": round(r["load_max"], 3), "seconds": round(secs, 3),
+            "host": host_cpu()}
 
 
 def main() -> int:
@@ -269,7 +326,7 @@ def main() -> int:
             log(f"{v}: {scores[v]}")
     head = results.get("bf16-persistent") or next(iter(results.values()))
     base = results.get("fp32-plain")
-    cpu = None if args.no_cpu else cpu_baseline(c, csr, args.cpu_seconds)
+    cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_seconds)
     line = {
         "metric": "Mwords/sec trained, BF16 persistent vs FP32 plain, at equal planted-cluster score",
         "value": round(head["mwords_s"], 2),

@@ -90,6 +90,10 @@ def lib(flavour: str = "parity") -> C.CDLL:
     L.orc_train.argtypes = [vp, vp, u64, vp, i32, u64, vp, vp, vp, vp, vp, vp, vp]
     L.orc_train_hogwild.restype = f64
     L.orc_train_hogwild.argtypes = [vp, vp, u64, vp, i32, u64, vp, vp, vp, vp, vp]
+    L.orc_train_hogwild_file.restype = f64
+    L.orc_train_hogwild_file.argtypes = [vp, vp, cp, u64, u64, vp, vp, vp, vp, vp, vp]
+    L.orc_vocab_file_size.restype = u64
+    L.orc_vocab_file_size.argtypes = [vp]
     for g in ("orc_gemm_nt", "orc_gemm_nn", "orc_gemm_tn"):
         getattr(L, g).argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, vp, C.c_int]
     L.orc_save_model.argtypes = [cp, vp, vp, i32, i32]
@@ -154,6 +158,18 @@ class OracleVocab:
     @classmethod
     def learn(cls, path: str, min_count: int = 5):
         return cls(lib().orc_vocab_learn(path.encode(), min_count))
+
+    @classmethod
+    def read(cls, vocab_path: str, train_path: str, min_count: int = 5, flavour: str = "parity"):
+        """ReadVocab (cpu:301-335) from a "word count" file; the training
+        file only gives the byte size the thread slices are cut from."""
+        v = cls(lib(flavour).orc_vocab_read(vocab_path.encode(), train_path.encode(), min_count))
+        v.flavour = flavour
+        return v
+
+    @property
+    def file_size(self) -> int:
+        return int(lib().orc_vocab_file_size(self.h))
 
     def __len__(self):
         return lib().orc_vocab_size(self.h)
@@ -264,9 +280,12 @@ def train(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, wi
 
 def train_hogwild(stream: np.ndarray, counts: np.ndarray, train_words: int, dim: int, window: int, negative: int,
                   batch_size: int, iters: int, alpha: float, sample: float, threads: int,
-                  table: np.ndarray | None = None) -> tuple[float, dict, np.ndarray]:
-    """OpenMP Hogwild run of the timing flavour; returns (seconds, stats, W_in)."""
-    L = lib("fast")
+                  table: np.ndarray | None = None, flavour: str = "fast",
+                  w_out_out: list | None = None) -> tuple[float, dict, np.ndarray]:
+    """OpenMP Hogwild run over an in-memory stream cut into `threads` equal
+    word slices; returns (training seconds, stats, W_in).  The timing flavour
+    by default; the parity flavour with one thread is orc_train's schedule."""
+    L = lib(flavour)
     counts = np.ascontiguousarray(counts, np.uint32)
     V = counts.size
     s = np.ascontiguousarray(stream, np.int32)
@@ -278,7 +297,55 @@ def train_hogwild(stream: np.ndarray, counts: np.ndarray, train_words: int, dim:
     st = Stats()
     secs = L.orc_train_hogwild(C.byref(p), P(s), s.size, P(counts), V, train_words, P(table), P(et), P(w_in),
                                P(w_out), C.byref(st))
+    if w_out_out is not None:
+        w_out_out.append(w_out)
     return secs, st.as_dict(), w_in
+
+
+def train_hogwild_file(vocab: OracleVocab, path: str, file_words: int, train_words: int, dim: int, window: int,
+                       negative: int, batch_size: int, iters: int, alpha: float, sample: float, threads: int,
+                       table: np.ndarray | None = None, flavour: str = "fast") -> dict:
+    """The Hogwild trainer as the reference's harness runs it (cpu:397-416):
+    thread `id` seeks to file_size * id / threads in the text file `path`,
+    loads up to ceil(file_words / threads) words and trains them; the time
+    covers loading and training (cpu:395, 702-706).  Returns seconds,
+    load_max (the longest per-thread load), stats, W_in and W_out."""
+    L = lib(flavour)
+    counts = vocab.counts()
+    V = counts.size
+    w_in, w_out = init_net(V, dim)
+    if table is None:
+        table = unigram_table(counts)
+    et = exp_table()
+    p = Params(dim, window, negative, batch_size, iters, threads, alpha, sample, 0, 0, 0, 0, 0)
+    st = Stats()
+    load = C.c_double(0.0)
+    secs = L.orc_train_hogwild_file(C.byref(p), vocab.h, path.encode(), file_words, train_words, P(table), P(et),
+                                    P(w_in), P(w_out), C.byref(st), C.byref(load))
+    if secs < 0:
+        raise RuntimeError(f"orc_train_hogwild_file: cannot read {path}")
+    return dict(seconds=secs, load_max=load.value, stats=st.as_dict(), w_in=w_in, w_out=w_out)
+
+
+def write_text_corpus(path: str, stream: np.ndarray, words: list[str]) -> int:
+    """Writes vocabulary-id `stream` as text: id 0 ("</s>") ends a line,
+    other ids are `words[id]` followed by a space.  Returns the byte size."""
+    enc = [w.encode() for w in words]
+    tok = np.array([b"\n"] + [e + b" " for e in enc[1:]], dtype=object)
+    n = 0
+    with open(path, "wb") as f:
+        for lo in range(0, stream.size, 1 << 20):
+            b = b"".join(tok[stream[lo:lo + (1 << 20)]].tolist())
+            f.write(b)
+            n += len(b)
+    return n
+
+
+def write_vocab(path: str, words: list[str], counts: np.ndarray) -> None:
+    """The "word count" lines ReadVocab (cpu:301-335) reads."""
+    with open(path, "w") as f:
+        for w, c in zip(words, counts.tolist()):
+            f.write(f"{w} {c}\n")
 
 
 def window_loss_f64(centre, ctx, neg, w_in: np.ndarray, w_out: np.ndarray) -> float:

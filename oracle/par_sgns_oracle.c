@@ -802,11 +802,89 @@ double orc_train_hogwild(const orc_params *p, const int32_t *stream, uint64_t n_
             stats->loss_rep += s->loss_rep;
             stats->draws += s->draws;
         }
+        stats->final_rng = ts[0].st.final_rng; /* thread 0's */
+        stats->final_alpha = ts[0].st.final_alpha;
     }
     for (int id = 0; id < nt; id++) free(slices[id]);
     free(slices);
     free(ts);
     return t1 - t0;
+}
+
+/* cpu:397-416 as the reference runs it: each of p->threads threads opens the
+ * text file, seeks to byte file_size * id / threads, loads up to
+ * ceil(file_words / threads) in-vocabulary words (loadStream, cpu:359-370;
+ * ReadWordIndex, cpu:146-196) and trains them with seed id; the returned wall
+ * time covers loading and training, like the reference's Elapsed
+ * (cpu:395, 702-706).  file_words slices the file (the reference uses
+ * train_words for both; a sample of a larger corpus keeps that corpus's
+ * train_words for subsampling and the learning rate).  *load_max gets the
+ * longest per-thread load time. */
+double orc_train_hogwild_file(const orc_params *p, const orc_vocab *v, const char *path, uint64_t file_words,
+                              uint64_t train_words, const int32_t *table, const float *exp_table,
+                              float *w_in, float *w_out, orc_stats *stats, double *load_max) {
+    int nt = p->threads > 0 ? p->threads : 1;
+    uint64_t per = file_words / (uint64_t)nt + (file_words % (uint64_t)nt > 0 ? 1 : 0); /* cpu:408 */
+    uint64_t wca = 0;
+    float a = p->alpha;
+    orc_thread *ts = (orc_thread *)calloc((size_t)nt, sizeof(orc_thread));
+    int32_t **streams = (int32_t **)calloc((size_t)nt, sizeof(int32_t *));
+    int failed = 0;
+    double lmax = 0.0;
+    double t0 = orc_now();
+#pragma omp parallel for num_threads(nt) schedule(static, 1) reduction(| : failed) reduction(max : lmax)
+    for (int id = 0; id < nt; id++) {
+        double l0 = orc_now();
+        streams[id] = (int32_t *)malloc((size_t)(per + 1) * sizeof(int32_t));
+        int64_t n = streams[id] ? orc_load_stream(v, path, v->file_size * (uint64_t)id / (uint64_t)nt, per, streams[id])
+                                : -1;
+        double l1 = orc_now();
+        if (l1 - l0 > lmax) lmax = l1 - l0;
+        if (n < 0) {
+            failed |= 1;
+            continue;
+        }
+        orc_thread *t = &ts[id];
+        t->p = p;
+        t->stream = streams[id];
+        t->local_words = (uint64_t)n;
+        t->cn = v->cn;
+        t->V = v->size;
+        t->T = train_words;
+        t->table = table;
+        t->et = exp_table;
+        t->w_in = w_in;
+        t->w_out = w_out;
+        t->rng = (uint64_t)id;
+        t->wc_actual = &wca;
+        t->alpha_shared = &a;
+        t->shared = 1;
+        orc_thread_run(t);
+    }
+    double t1 = orc_now();
+    if (load_max) *load_max = lmax;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        for (int id = 0; id < nt; id++) {
+            orc_stats *s = &ts[id].st;
+            stats->raw_words += s->raw_words;
+            stats->trained_tokens += s->trained_tokens;
+            stats->sentences += s->sentences;
+            stats->pairs += s->pairs;
+            stats->batches += s->batches;
+            stats->negatives += s->negatives;
+            stats->rep_pairs += s->rep_pairs;
+            stats->loss_attr += s->loss_attr;
+            stats->loss_rep += s->loss_rep;
+            stats->draws += s->draws;
+        }
+        stats->final_rng = ts[0].st.final_rng; /* thread 0's */
+        stats->final_alpha = ts[0].st.final_alpha;
+    }
+    for (int id = 0; id < nt; id++) free(streams[id]);
+    free(streams);
+    free(ts);
+    return failed ? -1.0 : t1 - t0;
 }
 
 /* cpu:721-737: header "V d\n", then per word "%s " + d * "%f " (or d raw

@@ -160,6 +160,14 @@ double orc_train_hogwild(const orc_params *p, const int32_t *stream, uint64_t n_
                          const int32_t *table, const float *exp_table,
                          float *w_in, float *w_out, orc_stats *stats);
 
+/* The Hogwild trainer as the reference's harness runs it (cpu:397-416):
+ * per-thread byte-offset slices of a text file, loading inside the timed
+ * region.  Returns the wall time of loading + training (-1 on error);
+ * *load_max (if given) gets the longest per-thread load time. */
+double orc_train_hogwild_file(const orc_params *p, const orc_vocab *v, const char *path, uint64_t file_words,
+                              uint64_t train_words, const int32_t *table, const float *exp_table,
+                              float *w_in, float *w_out, orc_stats *stats, double *load_max);
+
 /* Row-major naive GEMMs with the oracle's fixed summation order. */
 void orc_gemm_nt(int m, int n, int k, const float *a, int lda, const float *b, int ldb, float *c, int ldc);
 void orc_gemm_nn(int m, int n, int k, const float *a, int lda, const float *b, int ldb, float *c, int ldc);
