@@ -63,6 +63,24 @@ int64_t ph_load_text_stream(const ph_vocab *v, const char *train_path, int32_t *
 /* Raw integer corpus -> id stream (pruned words skipped, line ends -> 0). */
 int64_t ph_map_raw_stream(const ph_vocab *v, const int32_t *raw, int64_t n, int32_t *out, int64_t cap);
 
+/* Binary corpus: the vocabulary (words, counts, train_words) and the id
+ * stream of a text corpus in one file, so a large corpus is tokenised once.
+ * The reference re-reads text every run (ReadWord + SearchVocab, cpu:120-196;
+ * LearnVocabFromTrainFile cpu:253-292; loadStream cpu:359-370); these three
+ * calls have no counterpart there.  Layout, little-endian: 64-byte header
+ * ("PSGNSBC1", header size 64, int32 V, uint64 train_words, int64 n_ids,
+ * uint64 words_bytes, uint64 ids_offset, 16 reserved bytes), uint32 counts[V],
+ * the V words NUL-terminated, zero padding to ids_offset (a multiple of 8),
+ * int32 ids[n_ids].  Ids keep the vocabulary's order, so a corpus saved from
+ * ph_vocab_learn_text + ph_load_text_stream loads back identical. */
+int ph_corpus_save(const char *path, const ph_vocab *v, const int32_t *stream, int64_t n);
+/* The vocabulary of a binary corpus (no re-sort or pruning); *n_ids (may be
+ * NULL) receives the stream length.  NULL on a missing or malformed file. */
+ph_vocab *ph_corpus_vocab(const char *path, int64_t *n_ids);
+/* The id stream of a binary corpus: with out NULL returns its length; else
+ * writes it (cap >= length, ERANGE otherwise) and checks every id < V. */
+int64_t ph_corpus_ids(const char *path, int32_t *out, int64_t cap);
+
 /* ------------------------------------------------------------ subsampling */
 /* One-pass subsampling into a sentence CSR for the GPU trainers (the role of
  * PAR_Word2Vec_gpu_text.cu:785-828; keep rule of cpu:458-463).  A sentence

@@ -94,6 +94,9 @@ HOST_SYMBOLS = {
                            vp, i64, vp]),
     "ph_synth_sbm_walks": (i64, [i32, i32, f64, f64, f64, i32, i32, u64, vp, i64, vp]),
     "ph_write_raw_text": (C.c_int, [cp, vp, i64]),
+    "ph_corpus_save": (C.c_int, [cp, vp, vp, i64]),
+    "ph_corpus_vocab": (vp, [cp, vp]),
+    "ph_corpus_ids": (i64, [cp, vp, i64]),
     "ph_save_embeddings": (C.c_int, [cp, vp, vp, i32, i32]),
 }
 

@@ -26,6 +26,7 @@ int arg_pos(const char *name, int argc, char **argv) {  // cpu:713-719
 
 struct Opts {
     std::string train, output, save_vocab, read_vocab, mode = "persistent", precision = "fp32", time_file;
+    std::string train_bin, save_bin;  // binary corpus (vocabulary + id stream): read instead of -train / write
     int size = 100, window = 5, negative = 5, iter = 5, min_count = 5, batch = 24, binary = 0, debug = 2;
     int device = 0;
     float alpha = 0.025f, sample = 1e-3f;
@@ -48,6 +49,8 @@ void usage() {
         "  -autotune <runs> (0): persistent mode, tune the launch shape on the first chunk first\n"
         "  -min-sentences-per-wave <int> (16): persistent mode, fewer waves on small corpora\n"
         "  -write-back atomic|store_out|store_in|store|hot (atomic) -hot-rows <int>: bf16 row write-back\n"
+        "  -train-bin <file>: binary corpus (vocabulary + ids) in place of -train / -read-vocab\n"
+        "  -save-bin <file>: write the binary corpus of -train (then train only if -output is given)\n"
         "  accepted and ignored: -threads -disk -gamma -max-num-sen\n");
 }
 
@@ -90,33 +93,53 @@ int main(int argc, char **argv) {
     str("-write-back", o.write_back);
     num("-hot-rows", o.hot_rows);
     str("-time-file", o.time_file);
+    str("-train-bin", o.train_bin);
+    str("-save-bin", o.save_bin);
     if ((i = arg_pos("-seed", argc, argv)) > 0 && i + 1 < argc) o.seed = std::strtoull(argv[i + 1], nullptr, 10);
     if ((i = arg_pos("-chunk-words", argc, argv)) > 0 && i + 1 < argc) o.chunk_words = std::atoll(argv[i + 1]);
     if (o.time_file.empty()) o.time_file = "PAR_Word2Vec_gpu_time";
     const int mode = o.mode == "det" ? PAR_SGNS_DETERMINISTIC : o.mode == "plain" ? PAR_SGNS_PLAIN : PAR_SGNS_PERSISTENT;
     if (o.mode != "det" && o.mode != "plain" && o.mode != "persistent") return fail("-mode", "det, plain or persistent");
     if (o.precision != "fp32" && o.precision != "bf16") return fail("-precision", "fp32 or bf16");
-    if (o.train.empty()) return fail("-train", "training file required");
+    if (o.train.empty() && o.train_bin.empty()) return fail("-train", "training file (or -train-bin) required");
+    if (!o.train.empty() && !o.train_bin.empty()) return fail("-train-bin", "give -train or -train-bin, not both");
+    if (!o.train_bin.empty() && !o.save_bin.empty()) return fail("-save-bin", "the corpus is binary already");
 
     std::printf("hidden size: %d\nwindow size: %d\nnumber of negative samples: %d\nbatch size: %d\n"
                 "number of iterations: %d\nstarting learning rate: %.5f\nmode: %s, precision: %s\n"
                 "starting training using file: %s\n\n",
                 o.size, o.window, o.negative, o.batch, o.iter, o.alpha, o.mode.c_str(), o.precision.c_str(),
-                o.train.c_str());
+                o.train_bin.empty() ? o.train.c_str() : o.train_bin.c_str());
 
-    // vocabulary (cpu:378-384) and stream (cpu:359-370)
-    ph_vocab *v = o.read_vocab.empty() ? ph_vocab_learn_text(o.train.c_str(), o.min_count)
-                                       : ph_vocab_read(o.read_vocab.c_str(), o.min_count);
+    // vocabulary (cpu:378-384) and stream (cpu:359-370), or both from a binary corpus
+    const auto l0 = std::chrono::steady_clock::now();
+    int64_t n = 0;
+    ph_vocab *v = !o.train_bin.empty()     ? ph_corpus_vocab(o.train_bin.c_str(), &n)
+                  : o.read_vocab.empty() ? ph_vocab_learn_text(o.train.c_str(), o.min_count)
+                                         : ph_vocab_read(o.read_vocab.c_str(), o.min_count);
     if (!v) return fail("vocabulary", ph_last_error());
     if (!o.save_vocab.empty() && ph_vocab_save(v, o.save_vocab.c_str()) < 0) return fail("save-vocab", ph_last_error());
     const int32_t V = ph_vocab_size(v);
     const uint64_t T = ph_vocab_train_words(v);
     if (o.debug > 0) std::printf("Vocab size: %d\nWords in train file: %llu\n", V, (unsigned long long)T);
-    if (o.output.empty()) return 0;  // cpu:385
-    int64_t n = ph_load_text_stream(v, o.train.c_str(), nullptr, 0);
-    if (n < 0) return fail("stream", ph_last_error());
-    std::vector<int32_t> stream((size_t)n);
-    if (ph_load_text_stream(v, o.train.c_str(), stream.data(), n) < 0) return fail("stream", ph_last_error());
+    if (o.output.empty() && o.save_bin.empty()) return 0;  // cpu:385
+    std::vector<int32_t> stream;
+    if (!o.train_bin.empty()) {
+        stream.resize((size_t)n);
+        if (ph_corpus_ids(o.train_bin.c_str(), stream.data(), n) < 0) return fail("stream", ph_last_error());
+    } else {
+        n = ph_load_text_stream(v, o.train.c_str(), nullptr, 0);
+        if (n < 0) return fail("stream", ph_last_error());
+        stream.resize((size_t)n);
+        if (ph_load_text_stream(v, o.train.c_str(), stream.data(), n) < 0) return fail("stream", ph_last_error());
+    }
+    if (o.debug > 0)
+        std::printf("corpus loaded: %lld ids in %.2f s\n", (long long)n,
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - l0).count());
+    if (!o.save_bin.empty()) {
+        if (ph_corpus_save(o.save_bin.c_str(), v, stream.data(), n) < 0) return fail("save-bin", ph_last_error());
+        if (o.output.empty()) return 0;
+    }
     std::vector<uint32_t> counts((size_t)V);
     ph_vocab_counts(v, counts.data());
 

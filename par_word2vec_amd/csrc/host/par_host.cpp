@@ -840,6 +840,116 @@ int64_t ph_synth_sbm_walks(int32_t N, int32_t C, double expo, double mean_deg, d
     return o;
 }
 
+// ------------------------------------------------------- binary corpus --
+// Layout (little-endian): a 64-byte header, counts[V] uint32, the V words
+// NUL-terminated, zero padding to a multiple of 8, ids[n_ids] int32.
+namespace {
+constexpr char kBinMagic[8] = {'P', 'S', 'G', 'N', 'S', 'B', 'C', '1'};
+struct BinHeader {
+    char magic[8];
+    uint32_t header_bytes;   // 64
+    int32_t vocab_size;
+    uint64_t train_words;
+    int64_t n_ids;
+    uint64_t words_bytes;    // the NUL-terminated words, without padding
+    uint64_t ids_offset;     // byte offset of ids[0], a multiple of 8
+    uint64_t reserved[2];
+};
+static_assert(sizeof(BinHeader) == 64, "binary corpus header is 64 bytes");
+
+int read_bin_header(FILE *f, BinHeader &h) {
+    if (std::fread(&h, sizeof h, 1, f) != 1) return fail(PH_EINVAL, "binary corpus: truncated header");
+    if (std::memcmp(h.magic, kBinMagic, 8) != 0 || h.header_bytes != sizeof(BinHeader))
+        return fail(PH_EINVAL, "binary corpus: bad magic (not a par_word2vec binary corpus)");
+    if (h.vocab_size < 1 || h.n_ids < 0 || h.ids_offset % 8 != 0 ||
+        h.ids_offset < sizeof(BinHeader) + 4 * (uint64_t)h.vocab_size + h.words_bytes)
+        return fail(PH_EINVAL, "binary corpus: inconsistent header");
+    return 0;
+}
+}  // namespace
+
+int ph_corpus_save(const char *path, const ph_vocab *v, const int32_t *stream, int64_t n) {
+    if (!path || !v || (!stream && n > 0) || n < 0) return fail(PH_EINVAL, "null argument");
+    const int32_t V = (int32_t)v->words.size();
+    for (int64_t i = 0; i < n; i++)
+        if (stream[i] < 0 || stream[i] >= V) return fail(PH_EINVAL, "stream id out of vocabulary range");
+    BinHeader h{};
+    std::memcpy(h.magic, kBinMagic, 8);
+    h.header_bytes = sizeof(BinHeader);
+    h.vocab_size = V;
+    h.train_words = v->train_words;
+    h.n_ids = n;
+    for (const std::string &w : v->words) h.words_bytes += w.size() + 1;
+    h.ids_offset = (sizeof(BinHeader) + 4 * (uint64_t)V + h.words_bytes + 7) / 8 * 8;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(PH_EIO, std::string("cannot create ") + path);
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 &&
+              std::fwrite(v->cn.data(), 4, (size_t)V, f) == (size_t)V;
+    for (size_t i = 0; ok && i < v->words.size(); i++)
+        ok = std::fwrite(v->words[i].c_str(), 1, v->words[i].size() + 1, f) == v->words[i].size() + 1;
+    static const char zeros[8] = {0};
+    size_t pad = (size_t)(h.ids_offset - sizeof(BinHeader) - 4 * (uint64_t)V - h.words_bytes);
+    ok = ok && std::fwrite(zeros, 1, pad, f) == pad;
+    ok = ok && (n == 0 || std::fwrite(stream, 4, (size_t)n, f) == (size_t)n);
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? 0 : fail(PH_EIO, std::string("write failed: ") + path);
+}
+
+ph_vocab *ph_corpus_vocab(const char *path, int64_t *n_ids) {
+    FILE *f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) { fail(PH_EIO, std::string("cannot open ") + (path ? path : "(null)")); return nullptr; }
+    BinHeader h{};
+    if (read_bin_header(f, h) < 0) { std::fclose(f); return nullptr; }
+    std::unique_ptr<ph_vocab> v(new (std::nothrow) ph_vocab);
+    std::vector<char> words((size_t)h.words_bytes);
+    if (v) v->cn.resize((size_t)h.vocab_size);
+    bool ok = v && std::fread(v->cn.data(), 4, (size_t)h.vocab_size, f) == (size_t)h.vocab_size &&
+              std::fread(words.data(), 1, words.size(), f) == words.size();
+    std::fclose(f);
+    if (!ok) { fail(PH_EINVAL, "binary corpus: truncated vocabulary"); return nullptr; }
+    uint64_t total = 0;
+    size_t p = 0;
+    for (int32_t i = 0; i < h.vocab_size; i++) {
+        const char *s = words.data() + p;
+        size_t len = strnlen(s, words.size() - p);
+        if (p + len >= words.size() || len == 0 || len > 98) {
+            fail(PH_EINVAL, "binary corpus: malformed word table");
+            return nullptr;
+        }
+        v->words.emplace_back(s, len);
+        p += len + 1;
+        total += v->cn[(size_t)i];
+    }
+    if (p != words.size() || total != h.train_words) {
+        fail(PH_EINVAL, "binary corpus: word table or counts do not match the header");
+        return nullptr;
+    }
+    v->train_words = h.train_words;
+    v->rebuild_map();
+    if (n_ids) *n_ids = h.n_ids;
+    return v.release();
+}
+
+int64_t ph_corpus_ids(const char *path, int32_t *out, int64_t cap) {
+    FILE *f = path ? std::fopen(path, "rb") : nullptr;
+    if (!f) return fail(PH_EIO, std::string("cannot open ") + (path ? path : "(null)"));
+    BinHeader h{};
+    if (read_bin_header(f, h) < 0) { std::fclose(f); return PH_EINVAL; }
+    if (!out) { std::fclose(f); return h.n_ids; }
+    if (h.n_ids > cap) { std::fclose(f); return fail(PH_ERANGE, "stream capacity too small"); }
+    bool ok = std::fseek(f, (long)h.ids_offset, SEEK_SET) == 0 &&
+              std::fread(out, 4, (size_t)h.n_ids, f) == (size_t)h.n_ids;
+    std::fclose(f);
+    if (!ok) return fail(PH_EINVAL, "binary corpus: truncated id stream");
+    std::atomic<int> bad(0);
+    const int32_t V = h.vocab_size;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < h.n_ids; i++)
+        if (out[i] < 0 || out[i] >= V) bad = 1;
+    if (bad) return fail(PH_EINVAL, "binary corpus: id out of vocabulary range");
+    return h.n_ids;
+}
+
 int ph_write_raw_text(const char *path, const int32_t *raw, int64_t n) {
     if (!path || !raw || n < 0) return fail(PH_EINVAL, "null argument");
     FILE *f = std::fopen(path, "wb");

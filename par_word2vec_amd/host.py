@@ -230,6 +230,23 @@ def synth_sbm_walks(n_nodes: int = 10_000, n_comm: int = 20, exponent: float = 2
     return out[:n], comm
 
 
+def save_corpus(path: str, vocab: Vocab, stream: np.ndarray) -> None:
+    """Binary corpus: vocabulary + id stream in one file (ph_corpus_save)."""
+    s = np.ascontiguousarray(stream, dtype=np.int32)
+    _chk(N.host().ph_corpus_save(path.encode(), vocab._h, N.ptr(s), s.size), "save corpus")
+
+
+def load_corpus(path: str) -> tuple[Vocab, np.ndarray]:
+    """(vocabulary, id stream) of a binary corpus written by save_corpus."""
+    import ctypes as C
+    n = C.c_int64(0)
+    h = N.host().ph_corpus_vocab(path.encode(), C.byref(n))
+    vocab = Vocab(h)
+    out = np.empty(n.value, dtype=np.int32)
+    _chk(N.host().ph_corpus_ids(path.encode(), N.ptr(out), out.size), "corpus ids")
+    return vocab, out
+
+
 def write_raw_text(path: str, raw: np.ndarray) -> None:
     raw = np.ascontiguousarray(raw, dtype=np.int32)
     _chk(N.host().ph_write_raw_text(path.encode(), N.ptr(raw), raw.size), "write text")

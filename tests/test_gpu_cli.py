@@ -87,3 +87,22 @@ def test_cli_min_sentences_per_wave(tmp_path):
     assert shapes["1"] > shapes["16"]
     r = subprocess.run(base + ["-min-sentences-per-wave", "-3"], capture_output=True, text=True, timeout=300)
     assert r.returncode != 0 and "min_sentences_per_wave" in r.stderr
+
+
+def test_cli_binary_corpus_trains_like_text(tmp_path):
+    """-train-bin on the binary corpus that -save-bin wrote from the text gives
+    the same embedding file byte for byte (deterministic mode)."""
+    p = _corpus(tmp_path)
+    b = tmp_path / "corpus.bin"
+    r = subprocess.run([str(CLI), "-train", p, "-save-bin", str(b)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    flags = ["-size", "64", "-window", "5", "-negative", "5", "-sample", "1e-3", "-iter", "1", "-binary", "1",
+             "-mode", "det", "-time-file", str(tmp_path / "t")]
+    outs = []
+    for src in (["-train", p], ["-train-bin", str(b)]):
+        out = tmp_path / f"vec{len(outs)}.bin"
+        r = subprocess.run([str(CLI), *src, "-output", str(out), *flags], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs.append(out.read_bytes())
+    assert outs[0] == outs[1]

@@ -1,12 +1,18 @@
 """Host library (libpar_host.so) against the reference semantics and the
 oracle: tokenizer, vocabulary, id streams, the exact random schedule of the
 deterministic mode, subsampling into the sentence CSR, unigram table, writer
-and synthetic corpora.  CPU only."""
+and synthetic corpora, the binary corpus format.  CPU only."""
+import os
+import subprocess
+from pathlib import Path
+
 import numpy as np
 import pytest
 
 import oracle as O
 from par_word2vec_amd import corpora, host as H
+
+CLI = Path(__file__).resolve().parent.parent / "par_word2vec_amd" / "lib" / "par_word2vec"
 
 
 def write(tmp_path, name, data: bytes):
@@ -199,3 +205,77 @@ def test_workloads_generate_at_small_scale():
     for name in ("zipf1m", "sbm"):
         c = corpora.make(name, scale=0.05)
         assert c.stream.size > 0 and c.train_words == c.stream.size
+
+
+# ---------------------------------------------------------------- binary corpus
+def _text_corpus(tmp_path, n=1_500_000, seed=5):
+    raw, _ = H.synth_zipf(n, 20_000, 1.0, lognorm=(3.0, 0.8), len_clip=(5, 2000), n_clusters=20, cluster_size=10,
+                          p_replace=0.2, seed=seed)
+    p = str(tmp_path / "corpus.txt")
+    H.write_raw_text(p, raw)
+    return p
+
+
+def test_binary_corpus_equals_text_path(tmp_path):
+    """ph_corpus_save + ph_corpus_vocab / ph_corpus_ids give back the text
+    path's vocabulary (words, order, counts, train_words) and id stream
+    (LearnVocabFromTrainFile + loadStream, cpu:253-292, 359-370)."""
+    p = _text_corpus(tmp_path)
+    v = H.Vocab.learn_text(p, 5)
+    s = H.load_text_stream(v, p)
+    b = str(tmp_path / "corpus.bin")
+    H.save_corpus(b, v, s)
+    v2, s2 = H.load_corpus(b)
+    assert len(v2) == len(v) and v2.train_words == v.train_words
+    assert np.array_equal(v2.counts(), v.counts())
+    assert all(v2.word(i) == v.word(i) for i in range(len(v)))
+    assert all(v2.lookup(v.word(i)) == i for i in range(0, len(v), 97))
+    assert np.array_equal(s2, s)
+    assert os.path.getsize(b) == 64 + 4 * len(v) + -(-sum(len(v.word(i)) + 1 for i in range(len(v))) // 8) * 8 \
+        + 4 * s.size
+
+
+def test_binary_corpus_cli_conversion(tmp_path):
+    """-save-bin writes the same file as ph_corpus_save on the text path, and
+    -train-bin reads it (vocabulary report only: no -output, no device)."""
+    p = _text_corpus(tmp_path, 300_000)
+    b = tmp_path / "cli.bin"
+    r = subprocess.run([str(CLI), "-train", p, "-save-bin", str(b)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    v = H.Vocab.learn_text(p, 5)
+    H.save_corpus(str(tmp_path / "py.bin"), v, H.load_text_stream(v, p))
+    assert b.read_bytes() == (tmp_path / "py.bin").read_bytes()
+    r = subprocess.run([str(CLI), "-train-bin", str(b)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert f"Vocab size: {len(v)}" in r.stdout and f"Words in train file: {v.train_words}" in r.stdout
+    r = subprocess.run([str(CLI), "-train", p, "-train-bin", str(b)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "not both" in r.stderr
+
+
+def test_binary_corpus_rejects_malformed_files(tmp_path):
+    p = _text_corpus(tmp_path, 100_000)
+    v = H.Vocab.learn_text(p, 5)
+    s = H.load_text_stream(v, p)
+    b = tmp_path / "c.bin"
+    H.save_corpus(str(b), v, s)
+    data = bytearray(b.read_bytes())
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"XXXXXXXX" + data[8:])
+    with pytest.raises(H.HostError, match="bad magic"):
+        H.load_corpus(str(bad))
+    bad.write_bytes(data[: len(data) - 4])
+    with pytest.raises(H.HostError, match="truncated id stream"):
+        H.load_corpus(str(bad))
+    ids_off = int(np.frombuffer(bytes(data[40:48]), np.uint64)[0])
+    d2 = bytearray(data)
+    d2[ids_off + 40: ids_off + 44] = np.int32(len(v)).tobytes()
+    bad.write_bytes(bytes(d2))
+    with pytest.raises(H.HostError, match="out of vocabulary range"):
+        H.load_corpus(str(bad))
+    d3 = bytearray(data)
+    d3[64:68] = np.uint32(int(np.frombuffer(bytes(data[64:68]), np.uint32)[0]) + 1).tobytes()
+    bad.write_bytes(bytes(d3))
+    with pytest.raises(H.HostError, match="do not match"):
+        H.load_corpus(str(bad))
+    with pytest.raises(H.HostError, match="out of vocabulary range"):
+        H.save_corpus(str(bad), v, np.array([0, len(v)], np.int32))
