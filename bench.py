@@ -35,6 +35,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -48,7 +49,9 @@ from par_word2vec_amd.device import SgnsDevice  # noqa: E402
 from par_word2vec_amd.trainer import chunk_bounds  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "sgns_fast_kernel_1b_traffic.json"
+
+def traffic_file(workload: str) -> Path:
+    return ROOT / "profiles" / f"sgns_fast_kernel_{workload}_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -157,19 +160,67 @@ def calibration_copy(mib: int) -> None:
     del src, dst
 
 
-def measured_traffic(variant: str) -> float | None:
+def measured_traffic(workload: str, write_back: str, variant: str) -> float | None:
     """HBM bytes per launch of the training kernel from the committed counter
     profile of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
-    calibration copy), or None -- also when the profile was taken on a
-    different build of the kernel's sources."""
+    calibration copy), or None -- also when the profile was taken with another
+    write-back or on a different build of the kernel's sources."""
     try:
-        t = json.loads(TRAFFIC_FILE.read_text())
+        t = json.loads(traffic_file(workload).read_text())
+        if t.get("workload") != f"{workload}/{write_back}":
+            return None
         if t.get("kernel_source") != profiling.kernel_source_digest():
             log("counter profile is of another kernel build: traffic not reported")
             return None
         return float(t["variants"][variant]["bytes_per_launch"])
     except (OSError, KeyError, ValueError, TypeError):
         return None
+
+
+def time_loaders(c, workdir: str) -> dict:
+    """Host corpus loading at the workload's size, outside any kernel timing:
+    the corpus as text ("w<id>" words, one line per sentence), vocabulary +
+    id stream by the text path (LearnVocabFromTrainFile + loadStream,
+    cpu:253-292, 359-370), then the binary corpus written and read back.
+    Both results are checked against the in-memory corpus.  The files are
+    read right after being written, so from the page cache."""
+    import shutil
+    d = tempfile.mkdtemp(prefix="sgns_load_", dir=workdir)
+    out = {}
+    try:
+        text, binp = os.path.join(d, "corpus.txt"), os.path.join(d, "corpus.bin")
+        t = time.perf_counter()
+        H.write_raw_text(text, c.raw)
+        out["text_write_s"] = round(time.perf_counter() - t, 2)
+        out["text_bytes"] = os.path.getsize(text)
+        log(f"loader: {out['text_bytes'] / 1e9:.2f} GB of text written in {out['text_write_s']} s")
+        t = time.perf_counter()
+        v = H.Vocab.learn_text(text, c.workload.min_count)
+        out["text_vocab_s"] = round(time.perf_counter() - t, 2)
+        t = time.perf_counter()
+        s = H.load_text_stream(v, text)
+        out["text_stream_s"] = round(time.perf_counter() - t, 2)
+        assert len(v) == len(c.vocab) and v.train_words == c.train_words and np.array_equal(s, c.stream)
+        log(f"loader: text path {out['text_vocab_s']} s vocabulary + {out['text_stream_s']} s stream")
+        t = time.perf_counter()
+        H.save_corpus(binp, v, s)
+        out["binary_write_s"] = round(time.perf_counter() - t, 2)
+        out["binary_bytes"] = os.path.getsize(binp)
+        del s, v
+        t = time.perf_counter()
+        v2, s2 = H.load_corpus(binp)
+        out["binary_load_s"] = round(time.perf_counter() - t, 2)
+        assert len(v2) == len(c.vocab) and np.array_equal(s2, c.stream)
+        n = int(c.stream.size)
+        out["ids"] = n
+        out["text_mwords_s"] = round(n / (out["text_vocab_s"] + out["text_stream_s"]) / 1e6, 2)
+        out["binary_mwords_s"] = round(n / out["binary_load_s"] / 1e6, 2)
+        out["threads"] = cpu_threads()
+        out["method"] = ("text: byte ranges tokenised on every thread, vocabulary in one pass, the id stream in "
+                         "two (count, then fill); binary: one sequential read, ids checked on every thread")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
 
 
 def quality(c, w_in, w_out, windows) -> dict:
@@ -223,7 +274,6 @@ def cpu_baseline(c, seconds: float) -> dict:
     The vocabulary keeps the whole corpus's counts, so subsampling and the
     learning-rate schedule are the full corpus's."""
     import shutil
-    import tempfile
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
@@ -283,6 +333,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--autotune", action="store_true",
                     help="run the launch-shape tuner in this process instead of using the saved tuning")
+    ap.add_argument("--loaders", metavar="DIR", default="",
+                    help="time the text and binary corpus loaders on the workload's corpus (files under DIR), "
+                         "print one JSON line and exit; no device use")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="run a device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -291,6 +344,10 @@ def main() -> int:
               f"(the multi-GPU path is a design note, not code)", file=sys.stderr)
         return 2
 
+    if args.loaders:
+        c = corpora.make(args.workload, args.scale)
+        print(json.dumps({"workload": c.workload.name, "loaders": time_loaders(c, args.loaders)}), flush=True)
+        return 0
     if args.calibrate_copy_mb > 0:
         calibration_copy(args.calibrate_copy_mb)
     t0 = time.perf_counter()
@@ -313,6 +370,12 @@ def main() -> int:
                                  workload=args.workload, tune=args.autotune,
                                  write_back=args.write_back if (prec == "bf16" and not rep) or
                                  args.write_back in ("atomic", "store_out") else "atomic")
+    for r in results.values():  # counter-measured HBM bytes (the default chunking's profile)
+        t = (measured_traffic(args.workload, r["write_back"], r["kernel"])
+             if args.scale == 1.0 and args.chunk_words == 10_000_000 else None)
+        if t is not None:
+            r["counter_bytes_per_launch"] = t
+            r["counter_gbs"] = t / (r["avg_launch_ms"] / 1e3) / 1e9
     scores = {}
     if not args.no_score:
         windows = None
@@ -352,10 +415,7 @@ def main() -> int:
                      for k, r in results.items()},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     # counter profile of the default workload and chunking only
-                     "traffic": (measured_traffic(head["kernel"])
-                                 if args.workload == "1b" and args.scale == 1.0 and args.chunk_words == 10_000_000
-                                 else None),
+                     "traffic": head.get("counter_bytes_per_launch"),
                      "algorithmic_bytes_per_launch": round(head["alg_bytes"] / max(head["launches"], 1)),
                      "kernel": f"sgns_fast_kernel ({head['kernel']} repulsion)",
                      "avg_launch_ms": round(head["avg_launch_ms"], 4)},

@@ -18,7 +18,9 @@
 // shared negatives are drawn from the on-device alias table with the
 // reference's in-sentence rejection; the K x B logits, the label-0 gradient
 // and both updates come from one snapshot, on the matrix cores (bf16 tables,
-// repulsion_mfma) or in register blocks of NB negatives (vector path).
+// repulsion_mfma) or in register blocks of NB negatives (vector path, which
+// sums each batch row's W_in delta over the blocks in LDS).  Either way every
+// W_in and W_out row of a batch is written back once.
 //
 // Write-back: fp32 tables take the delta with global float atomics (no lost
 // updates, as the reference GPU's atomicAdd, text.cu:438-439, 710-717);
@@ -943,15 +945,18 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     // ---- repulsion (cpu:571-679)
     // Per batch of B positions: K shared negatives; logits G = N C^T against
     // the batch rows C, label-0 gradient, then W_out[n_k] += sum_i G_ki c_i and
-    // W_in[c_i] += sum_k G_ki n_k, all from one snapshot as the reference's
-    // three cblas_sgemm calls (cpu:633-656).  The batch rows are snapshotted
-    // into the (now idle) ring memory; repeated negatives are merged with a
-    // multiplicity so every word's W_out row is read and written by exactly
-    // one register block of NB negatives.
+    // W_in[c_i] += sum_k G_ki n_k, as the reference's three cblas_sgemm calls
+    // (cpu:633-656).  Repeated negatives are merged with a multiplicity so
+    // every word's W_out row is read and written by exactly one register block
+    // of NB negatives.  Vector path: each block re-reads the batch rows C
+    // (cache-resident) and adds its share of W_in[c_i]'s delta to an fp32
+    // accumulator in the (now idle) ring memory; the rows are written once per
+    // batch, after the last block (the reference's one scatter per batch,
+    // cpu:658-674).  With no concurrent writer -- one wave -- every block
+    // reads the same C, the reference's snapshot.
     const int B = a.B, NEG = a.NEG;
     const int nbat = (L + B - 1) / B;
     const float alpha_r = L == 1 ? alpha0 : alpha;
-    PT *snap = reinterpret_cast<PT *>(lds);
     int32_t *uneg = negs + a.kmax;  // unique negatives of the batch
     int32_t *umul = uneg + a.kmax;  // their multiplicities
     for (int bi = 0; bi < nbat; bi++) {
@@ -1027,20 +1032,12 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             repulsion_mfma<NP, WIN, WOUT>(a, sen + lo, isz, U, uneg, umul, lds, lane, alpha_r, loss_r, rs);
             continue;
         }
-        // snapshot of the batch rows (after this sentence's attraction), four in flight
-        for (int i0 = 0; i0 < isz; i0 += 4) {
-            PT rows[4][NP];
+        float2 *acc = reinterpret_cast<float2 *>(lds);  // [isz][nq] fp32 W_in deltas
+        for (int i = 0; i < isz; i++) {
 #pragma unroll
-            for (int k = 0; k < 4; k++)
-                if (i0 + k < isz) load_row<T, NP>(w_in, sen[lo + i0 + k], ld, nq, lane, rows[k]);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (i0 + k >= isz) continue;
-#pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    if (t < NP - 1 || q < nq) snap[(i0 + k) * nq + q] = rows[k][t];
-                }
+            for (int t = 0; t < NP; t++) {
+                const int q = lane + kWave * t;
+                if (t < NP - 1 || q < nq) acc[i * nq + q] = make_float2(0.0f, 0.0f);
             }
         }
         // W_out rows of a register block; the next block's rows are loaded
@@ -1071,14 +1068,18 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
 #pragma unroll
                 for (int t = 0; t < NP; t++) dn[k][t] = make_float2(0.0f, 0.0f);
             }
+            PT ra[NP], rb[NP];
+            load_row<T, NP>(w_in, sen[lo], ld, nq, lane, ra);
             if (kb + NB < U) load_block(kb + NB);
             const float mlane = lane < nbk ? (float)umul[kb + lane] : 0.0f;  // loss: lane k <-> negative k
             for (int i = 0; i < isz; i++) {
                 float2 c[NP];
-#pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    c[t] = t < NP - 1 || q < nq ? Pair<T>::tof(snap[i * nq + q]) : make_float2(0.0f, 0.0f);
+                if ((i & 1) == 0) {
+                    if (i + 1 < isz) load_row<T, NP>(w_in, sen[lo + i + 1], ld, nq, lane, rb);
+                    to_f<T, NP>(ra, c);
+                } else {
+                    if (i + 1 < isz) load_row<T, NP>(w_in, sen[lo + i + 1], ld, nq, lane, ra);
+                    to_f<T, NP>(rb, c);
                 }
                 float g[NB];
                 float fbuf = 0.0f;  // lane k: logit against negative k (loss only)
@@ -1105,7 +1106,16 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                         dn[k][t].y += g[k] * c[t].y;
                     }
                 }
-                store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, c, dc, rs, a.rounding);
+#pragma unroll
+                for (int t = 0; t < NP; t++) {
+                    const int q = lane + kWave * t;
+                    if (t < NP - 1 || q < nq) {
+                        float2 v = acc[i * nq + q];
+                        v.x += dc[t].x;
+                        v.y += dc[t].y;
+                        acc[i * nq + q] = v;
+                    }
+                }
             }
 #pragma unroll
             for (int k = 0; k < NB; k++) {
@@ -1118,6 +1128,23 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                     store_delta<T, NP, WOUT == kWbStore>(w_out, nrow[k], ld, nq, lane, nv[k], dn[k], rs, a.rounding);
                 }
             }
+        }
+        for (int i = 0; i < isz; i++) {
+            float2 c[NP], dc[NP];
+            if constexpr (sizeof(T) == 2) {
+                PT r[NP];
+                load_row<T, NP>(w_in, sen[lo + i], ld, nq, lane, r);
+                to_f<T, NP>(r, c);
+            } else {
+#pragma unroll
+                for (int t = 0; t < NP; t++) c[t] = make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int t = 0; t < NP; t++) {
+                const int q = lane + kWave * t;
+                dc[t] = t < NP - 1 || q < nq ? acc[i * nq + q] : make_float2(0.0f, 0.0f);
+            }
+            store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, c, dc, rs, a.rounding);
         }
     }
 }

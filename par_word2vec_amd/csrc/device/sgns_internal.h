@@ -120,7 +120,7 @@ inline int mf_bytes(int ld, int B) {
 // Offset of the negatives = the snapshot's size.
 inline int fast_lds_negs(int nq, int bf16, int B, int ld, int mfma) {
     const int e = bf16 ? 4 : 8;
-    int snap = B * nq * e;
+    int snap = B * nq * (mfma ? e : 8);  // matrix path: the batch rows; vector path: their fp32 W_in deltas
     if (mfma && mf_bytes(ld, B) > snap) snap = mf_bytes(ld, B);
     return (snap + 15) & ~15;
 }

@@ -103,19 +103,28 @@ struct Alias {
 // reference's end-of-file rule: a word cut off by EOF sets feof and is
 // discarded by every caller (cpu:172, 269).  Words keep their first 98 bytes
 // (MAX_STRING 100, cpu:140-143) and end at an embedded NUL like a C string.
+// The bytes [begin, end) of the file; a range that starts right after a
+// space, tab or newline (or at 0) yields exactly the tokens one pass over the
+// whole file yields there.
 template <class F>
-int for_each_token(const char *path, F &&emit) {
+int for_each_token_range(const char *path, int64_t begin, int64_t end, F &&emit) {
     FILE *f = std::fopen(path, "rb");
     if (!f) return fail(PH_EIO, std::string("cannot open ") + path);
+    if (begin > 0 && std::fseek(f, (long)begin, SEEK_SET) != 0) {
+        std::fclose(f);
+        return fail(PH_EIO, std::string("cannot seek in ") + path);
+    }
     const size_t BUF = 1 << 22;
     std::unique_ptr<unsigned char[]> buf(new (std::nothrow) unsigned char[BUF]);
     if (!buf) { std::fclose(f); return fail(PH_ENOMEM, "tokenizer buffer"); }
     char word[100];
     int a = 0;
     static const char EOS[] = "</s>";
-    for (;;) {
-        size_t got = std::fread(buf.get(), 1, BUF, f);
+    int64_t left = end < 0 ? INT64_MAX : end - begin;
+    while (left > 0) {
+        size_t got = std::fread(buf.get(), 1, (size_t)std::min<int64_t>((int64_t)BUF, left), f);
         if (got == 0) break;
+        left -= (int64_t)got;
         for (size_t k = 0; k < got; k++) {
             int ch = buf[k];
             if (ch == 13) continue;
@@ -134,6 +143,43 @@ int for_each_token(const char *path, F &&emit) {
     }
     std::fclose(f);
     return 0;  // a pending word (a > 0) at EOF is dropped
+}
+
+template <class F>
+int for_each_token(const char *path, F &&emit) {
+    return for_each_token_range(path, 0, -1, emit);
+}
+
+// Cuts the file into at most `parts` byte ranges for for_each_token_range:
+// each cut lies right after the first space, tab or newline at or past an
+// equal share of the bytes.  Returns the boundaries (first 0, last the file
+// size), or an empty vector if the file cannot be read.
+std::vector<int64_t> token_ranges(const char *path, int parts) {
+    std::vector<int64_t> cut;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return cut;
+    std::fseek(f, 0, SEEK_END);
+    const int64_t size = (int64_t)std::ftell(f);
+    const int64_t kMinPart = 1 << 20;  // smaller files: one range
+    if (parts > size / kMinPart) parts = (int)std::max<int64_t>(1, size / kMinPart);
+    cut.push_back(0);
+    unsigned char tmp[4096];
+    for (int k = 1; k < parts; k++) {
+        int64_t p = std::max(size * k / parts, cut.back());
+        int64_t at = size;
+        std::fseek(f, (long)p, SEEK_SET);
+        while (p < size && at == size) {
+            size_t got = std::fread(tmp, 1, sizeof tmp, f);
+            if (got == 0) break;
+            for (size_t i = 0; i < got; i++)
+                if (tmp[i] == ' ' || tmp[i] == '\t' || tmp[i] == '\n') { at = p + (int64_t)i + 1; break; }
+            p += (int64_t)got;
+        }
+        if (at > cut.back() && at < size) cut.push_back(at);
+    }
+    cut.push_back(size);
+    std::fclose(f);
+    return cut;
 }
 
 // Open-addressing string -> id map (FNV-1a); the reference's own hash only
@@ -155,6 +201,7 @@ struct WordMap {
         words = w;
     }
     int32_t find(const char *s, size_t n) const {
+        if (slot.empty()) return -1;
         size_t i = h(s, n) & mask;
         for (;;) {
             int32_t k = slot[i];
@@ -244,6 +291,48 @@ ph_vocab *ph_vocab_learn_text(const char *path, int32_t min_count) {
     v->add("</s>", 4);
     uint32_t min_reduce = 1;
     const size_t reduce_at = (size_t)(30000000 * 0.7);  // cpu:74, 282
+    // Byte ranges counted in parallel, each with its words in first-appearance
+    // order, then merged in file order: the same words, counts and first-
+    // appearance order as one pass -- as long as the one pass never reaches
+    // ReduceVocab's bound, i.e. the merged vocabulary stays within it;
+    // otherwise the one pass below runs.
+    const std::vector<int64_t> cut = token_ranges(path, omp_get_max_threads());
+    if (cut.size() > 2) {
+        const int parts = (int)cut.size() - 1;
+        std::vector<ph_vocab> local((size_t)parts);
+        std::vector<int> rcs((size_t)parts, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int k = 0; k < parts; k++) {
+            ph_vocab &lv = local[(size_t)k];
+            rcs[(size_t)k] = for_each_token_range(path, cut[(size_t)k], cut[(size_t)k + 1], [&](const char *s, size_t n) {
+                if (lv.words.size() > reduce_at) return;  // the merged vocabulary is past the bound too
+                int32_t i = lv.map.find(s, n);
+                if (i < 0) { i = lv.add(s, n); lv.cn[(size_t)i] = 1; }
+                else lv.cn[(size_t)i]++;
+            });
+        }
+        for (int k = 0; k < parts; k++)
+            if (rcs[(size_t)k] < 0) return nullptr;
+        bool over = false;
+        for (int k = 0; k < parts && !over; k++) {
+            const ph_vocab &lv = local[(size_t)k];
+            over = lv.words.size() > reduce_at;
+            for (size_t j = 0; j < lv.words.size() && !over; j++) {
+                const std::string &w = lv.words[j];
+                int32_t i = v->map.find(w.data(), w.size());
+                if (i < 0) i = v->add(w.data(), w.size());
+                v->cn[(size_t)i] += lv.cn[j];
+                over = v->words.size() > reduce_at;
+            }
+        }
+        if (!over) {
+            v->sort_prune(min_count);
+            return v.release();
+        }
+        v.reset(new (std::nothrow) ph_vocab);
+        if (!v) { fail(PH_ENOMEM, "vocab"); return nullptr; }
+        v->add("</s>", 4);
+    }
     int rc = for_each_token(path, [&](const char *s, size_t n) {
         int32_t i = v->map.find(s, n);
         if (i < 0) { i = v->add(s, n); v->cn[(size_t)i] = 1; }
@@ -344,15 +433,40 @@ int ph_vocab_raw_map(const ph_vocab *v, int32_t *out, int32_t raw_vocab) {
 
 int64_t ph_load_text_stream(const ph_vocab *v, const char *path, int32_t *out, int64_t cap) {
     if (!v || !path) return fail(PH_EINVAL, "null argument");
-    int64_t n = 0;
-    int rc = for_each_token(path, [&](const char *s, size_t len) {
-        int32_t id = v->map.find(s, len);
-        if (id < 0) return;
-        if (out && n < cap) out[n] = id;
-        n++;
-    });
-    if (rc < 0) return rc;
-    if (out && n > cap) return fail(PH_ERANGE, "stream capacity too small");
+    // byte ranges tokenised in parallel, each into its own id list, then
+    // concatenated in file order
+    std::vector<int64_t> cut = token_ranges(path, omp_get_max_threads());
+    if (cut.empty()) return fail(PH_EIO, std::string("cannot open ") + path);
+    const int parts = (int)cut.size() - 1;
+    std::vector<std::vector<int32_t>> ids((size_t)parts);
+    std::vector<int64_t> cnt((size_t)parts + 1, 0);
+    std::vector<int> rcs((size_t)parts, 0);
+    std::atomic<int> oom(0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < parts; k++) {
+        std::vector<int32_t> &o = ids[(size_t)k];
+        int64_t n = 0;
+        rcs[(size_t)k] = for_each_token_range(path, cut[(size_t)k], cut[(size_t)k + 1], [&](const char *s, size_t len) {
+            int32_t id = v->map.find(s, len);
+            if (id < 0) return;
+            n++;
+            if (out && !oom) {
+                try { o.push_back(id); } catch (const std::bad_alloc &) { oom = 1; }
+            }
+        });
+        cnt[(size_t)k + 1] = n;
+    }
+    for (int k = 0; k < parts; k++)
+        if (rcs[(size_t)k] < 0) return rcs[(size_t)k];
+    if (oom) return fail(PH_ENOMEM, "stream buffers");
+    for (int k = 0; k < parts; k++) cnt[(size_t)k + 1] += cnt[(size_t)k];
+    const int64_t n = cnt[(size_t)parts];
+    if (!out) return n;
+    if (n > cap) return fail(PH_ERANGE, "stream capacity too small");
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < parts; k++)
+        if (!ids[(size_t)k].empty())
+            std::memcpy(out + cnt[(size_t)k], ids[(size_t)k].data(), ids[(size_t)k].size() * sizeof(int32_t));
     return n;
 }
 

@@ -2,7 +2,7 @@
 committed profile notes and bench.py's roofline use).
 
     python -m par_word2vec_amd.profiling stats  <kernel_stats.csv>
-    python -m par_word2vec_amd.profiling traffic <fetch counter csv> <write counter csv> <copy MiB> <out.json>
+    python -m par_word2vec_amd.profiling traffic <fetch counter csv> <write counter csv> <copy MiB> <out.json> [workload/write-back]
 
 `stats` prints the per-kernel dispatch count and average duration of a
 `rocprofv3 --kernel-trace --stats --output-format csv` run.  `traffic` takes
@@ -10,7 +10,9 @@ two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per dispatch) of a
 command that first copies `copy MiB` device to device (bench.py
 --calibrate-copy-mb), scales each counter by what it reads for that copy
 (bytes per counted KiB) and writes bytes per launch for every training-kernel
-variant."""
+variant, with a tag naming the workload and write-back (default
+"1b/store_out") and a digest of the kernel's sources, so a profile of another
+workload, write-back or kernel build is not used."""
 from __future__ import annotations
 
 import csv
@@ -21,7 +23,9 @@ from collections import defaultdict
 from pathlib import Path
 
 COPY_KERNEL = "neg_kernel"  # torch.neg of the calibration buffer (bench.py --calibrate-copy-mb)
-KERNEL_SOURCES = ("fast_kernel.hip", "sgns_device.h", "sgns_internal.h")
+KERNEL_SOURCES = ("fast_kernel.h", "fast_kernel.hip", "fast_inst_mfma_atomic.hip", "fast_inst_mfma_hot.hip",
+                  "fast_inst_mfma_store.hip", "fast_inst_mfma_store_out.hip", "fast_inst_vec_bf16.hip",
+                  "fast_inst_vec_fp32.hip", "sgns_device.h", "sgns_internal.h")
 
 
 def kernel_source_digest() -> str:
@@ -62,9 +66,10 @@ def counter_means(path: str, counter: str) -> dict[str, tuple[int, float]]:
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
-def traffic(fetch_csv: str, write_csv: str, copy_mib: float) -> dict:
+def traffic(fetch_csv: str, write_csv: str, copy_mib: float, workload: str = "1b/store_out") -> dict:
     copy_bytes = copy_mib * (1 << 20)
-    out = {"kernel_source": kernel_source_digest(), "copy_bytes": copy_bytes, "counters": {}, "variants": {}}
+    out = {"workload": workload, "kernel_source": kernel_source_digest(), "copy_bytes": copy_bytes, "counters": {},
+           "variants": {}}
     per_kernel: dict[str, dict] = defaultdict(dict)
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         means = counter_means(path, name)
@@ -91,8 +96,8 @@ def main(argv: list[str]) -> int:
         for r in kernel_stats(argv[1]):
             print(f"{r['calls']:8d} {r['avg_ms']:12.4f} ms avg {r['total_ms']:12.2f} ms total  {r['name'][:110]}")
         return 0
-    if len(argv) == 5 and argv[0] == "traffic":
-        t = traffic(argv[1], argv[2], float(argv[3]))
+    if len(argv) in (5, 6) and argv[0] == "traffic":
+        t = traffic(argv[1], argv[2], float(argv[3]), *argv[5:6])
         with open(argv[4], "w") as f:
             json.dump(t, f, indent=1)
         print(json.dumps(t, indent=1))

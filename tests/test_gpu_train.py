@@ -211,10 +211,13 @@ def test_matrix_repulsion_exact_schedule(dim, window, negative, batch, single):
     sums, the gradient rounded to bf16 for dC and dN, one write-back per row
     and batch) against the fp32 deterministic run: its error must stay at the
     bf16 storage scale (RMS within 8% of the RMS update: round-to-nearest
-    drops updates below half an ulp, the dominant error of bf16 tables) and be
-    no larger than that of the vector repulsion on the same bf16 tables, which
-    does the same arithmetic in fp32 but rounds a row once per block of
-    negatives.  Losses are evaluated on the bf16 tables, which differ from the
+    drops updates below half an ulp, the dominant error of bf16 tables) and
+    within 1.25 sqrt(g) of that of the vector repulsion on the same bf16
+    tables, which does the same arithmetic in fp32 and rounds each W_in row
+    once per batch, where the matrix path rounds it once per group of 64
+    unique negatives, up to g = ceil(2 W negative / 64) times (independent
+    rounding errors add in quadrature), and also rounds the logit gradient to
+    bf16.  Losses are evaluated on the bf16 tables, which differ from the
     fp32 run at that storage scale.  With one-word sentences every update of a
     frequent negative is a write-back of its own, so W_out's storage error is
     that of round-to-nearest alone and equal for both paths (so are the
@@ -239,7 +242,8 @@ def test_matrix_repulsion_exact_schedule(dim, window, negative, batch, single):
         print(f"  {name}: rms update {upd:.3e}, matrix err {em:.3e}, vector err {ev:.3e}", flush=True)
         if not (single and name == "W_out"):
             assert em <= 0.08 * upd
-        assert em <= ev + 1e-3 * upd
+        g = -(-2 * window * negative // 64)
+        assert em <= 1.25 * np.sqrt(g) * ev + 1e-3 * upd
     for k in ("loss_attr", "loss_rep"):
         print(f"  {k}: det {ds[k]:.6e} matrix {ms[k]:.6e} vector {vs[k]:.6e}", flush=True)
         if single:  # storage-dominated: as close to fp32 as the vector path, within 5%

@@ -279,3 +279,26 @@ def test_binary_corpus_rejects_malformed_files(tmp_path):
         H.load_corpus(str(bad))
     with pytest.raises(H.HostError, match="out of vocabulary range"):
         H.save_corpus(str(bad), v, np.array([0, len(v)], np.int32))
+
+
+def test_parallel_text_loader_equals_oracle(tmp_path):
+    """Files above 1 MiB are tokenised in byte ranges on several threads.  On
+    a 6 MB file with every tokenizer case at random places (tabs, runs of
+    separators, "\\r\\n", blank lines, words longer than 98 bytes, a last
+    word cut off by EOF) the vocabulary, counts, first-appearance order and
+    id stream equal the oracle's single pass (cpu:120-144, 253-292, 359-370)."""
+    rng = np.random.default_rng(11)
+    words = [f"w{i}".encode() for i in range(3000)] + [b"L" * 120 + str(i).encode() for i in range(20)]
+    p = np.arange(1, len(words) + 1, dtype=np.float64) ** -1.0
+    tok = rng.choice(len(words), 1_400_000, p=p / p.sum())
+    seps = [b" ", b"\t", b"  ", b"\n", b"\r\n", b" \n\n", b"\t \t"]
+    sp = rng.choice(len(seps), tok.size, p=[0.6, 0.1, 0.1, 0.1, 0.05, 0.03, 0.02])
+    data = b"".join(words[t] + seps[s] for t, s in zip(tok.tolist(), sp.tolist())) + b"tail"
+    path = write(tmp_path, "adv.txt", data)
+    assert len(data) > 6_000_000
+    v = H.Vocab.learn_text(path, 3)
+    ov = O.OracleVocab.learn(path, 3)
+    assert len(v) == len(ov)
+    assert all(v.word(i) == ov.word(i) for i in range(len(v)))
+    assert np.array_equal(v.counts(), ov.counts()) and v.train_words == ov.train_words
+    assert np.array_equal(H.load_text_stream(v, path), ov.stream(path))
