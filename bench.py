@@ -160,14 +160,14 @@ def calibration_copy(mib: int) -> None:
     del src, dst
 
 
-def measured_traffic(workload: str, write_back: str, variant: str) -> float | None:
+def measured_traffic(workload: str, write_back: str, chunk_words: int, variant: str) -> float | None:
     """HBM bytes per launch of the training kernel from the committed counter
     profile of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by the
     calibration copy), or None -- also when the profile was taken with another
-    write-back or on a different build of the kernel's sources."""
+    write-back or chunking, or on a different build of the kernel's sources."""
     try:
         t = json.loads(traffic_file(workload).read_text())
-        if t.get("workload") != f"{workload}/{write_back}":
+        if t.get("workload") != f"{workload}/{write_back}/{chunk_words}":
             return None
         if t.get("kernel_source") != profiling.kernel_source_digest():
             log("counter profile is of another kernel build: traffic not reported")
@@ -371,8 +371,7 @@ def main() -> int:
                                  write_back=args.write_back if (prec == "bf16" and not rep) or
                                  args.write_back in ("atomic", "store_out") else "atomic")
     for r in results.values():  # counter-measured HBM bytes (the default chunking's profile)
-        t = (measured_traffic(args.workload, r["write_back"], r["kernel"])
-             if args.scale == 1.0 and args.chunk_words == 10_000_000 else None)
+        t = measured_traffic(args.workload, r["write_back"], args.chunk_words, r["kernel"]) if args.scale == 1.0 else None
         if t is not None:
             r["counter_bytes_per_launch"] = t
             r["counter_gbs"] = t / (r["avg_launch_ms"] / 1e3) / 1e9

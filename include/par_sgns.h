@@ -103,9 +103,13 @@ typedef struct par_sgns_train_args {
     int64_t sent_end;     /* (deterministic mode: of the uploaded plan) */
     int32_t sentences_per_dequeue; /* persistent mode: queue slots a wavefront takes per
                                       dequeue, 1..64; 0 = tuned default (1 untuned) */
-    int32_t min_sentences_per_wave; /* persistent mode, default grid: at least this many
-                                       sentences per wavefront (fewer waves on small ranges:
-                                       Hogwild staleness); 0 = 16 */
+    int32_t min_sentences_per_wave; /* Hogwild staleness bound.  Persistent mode, default
+                                       grid: at least this many sentences per wavefront
+                                       (fewer waves on small ranges); 0 = 16.  Plain mode,
+                                       if > 0: at most range / this many sentences in flight
+                                       (consecutive launches of that many sentences where
+                                       fewer than that fit on the device at once); 0 = one
+                                       launch over the range */
     int32_t write_back;   /* par_sgns_write_back; the vector repulsion (fp32 tables, or
                              bf16 on request) accepts ATOMIC and STORE_OUT */
     int32_t hot_rows;     /* PAR_SGNS_WB_HOT: rows with id < hot_rows keep their atomics */

@@ -687,10 +687,30 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
             HIPCHK(h, hipEventRecord(h->ev1, h->st));
         } else {
             if (prepare_only) return 0;
+            // Plain mode's optional staleness bound (min_sentences_per_wave >
+            // 0): at most n / floor sentences in flight.  Where that is fewer
+            // than fit on the device the range runs as consecutive launches
+            // of that many sentences -- bulk-synchronous rounds.  Without it
+            // every sentence of the range is one block of one launch, like
+            // the reference GPU kernel's grid (text.cu:887-912).
+            const int64_t n = a->sent_end - a->sent_begin;
+            const int64_t resident = (int64_t)blocks_per_cu(f.lds_per_wave, wpb) * wpb * (h->cu_count > 0 ? h->cu_count : 256);
+            int64_t part = n;
+            if (!pers && a->min_sentences_per_wave > 0) {
+                part = (n + a->min_sentences_per_wave - 1) / a->min_sentences_per_wave;
+                if (part >= resident || part < 1) part = n;
+            }
             HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, h->queue, h->st));
             HIPCHK(h, hipEventRecord(h->ev0, h->st));
-            HIPCHK(h, sgns::launch_fast(f, h->bf16, h->np, wpb, blocks, h->st));
+            launches = 0;
+            for (int64_t b = a->sent_begin; b < a->sent_end; b += part) {
+                f.s_begin = b;
+                f.s_end = b + part < a->sent_end ? b + part : a->sent_end;
+                HIPCHK(h, sgns::launch_fast(f, h->bf16, h->np, wpb, blocks, h->st));
+                launches++;
+            }
             HIPCHK(h, hipEventRecord(h->ev1, h->st));
+            if (!pers) shape_blocks = (int)((part + wpb - 1) / wpb);
         }
     } else {
         return seterr(h, PAR_SGNS_EINVAL, "unknown mode %d", a->mode);

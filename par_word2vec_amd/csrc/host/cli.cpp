@@ -33,7 +33,7 @@ struct Opts {
     unsigned long long seed = 1;
     long long chunk_words = 10000000;
     int autotune = 0;  // persistent mode: tune the launch shape on the first chunk, this many runs per shape
-    int min_sent_wave = 0;  // persistent mode: staleness floor of the default grid (0 = 16)
+    int min_sent_wave = 0;  // staleness bound: persistent grid floor (0 = 16); plain mode: sentences in flight <= n / this (0 = off)
     std::string write_back = "atomic";  // bf16 matrix-core write-back policy
     int hot_rows = 0;                   // write_back hot: rows with id < this keep atomics
 };
@@ -47,7 +47,8 @@ void usage() {
         "  -mode det|plain|persistent (persistent) -precision fp32|bf16 (fp32) -seed <int> -device <int>\n"
         "  -chunk-words <int> (10000000) -time-file <file> (PAR_Word2Vec_gpu_time)\n"
         "  -autotune <runs> (0): persistent mode, tune the launch shape on the first chunk first\n"
-        "  -min-sentences-per-wave <int> (16): persistent mode, fewer waves on small corpora\n"
+        "  -min-sentences-per-wave <int>: staleness bound; persistent mode: sentences per wave (16);\n"
+        "    plain mode: at most n / this sentences in flight (0: off)\n"
         "  -write-back atomic|store_out|store_in|store|hot (atomic) -hot-rows <int>: bf16 row write-back\n"
         "  -train-bin <file>: binary corpus (vocabulary + ids) in place of -train / -read-vocab\n"
         "  -save-bin <file>: write the binary corpus of -train (then train only if -output is given)\n"

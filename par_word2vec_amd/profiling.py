@@ -2,7 +2,7 @@
 committed profile notes and bench.py's roofline use).
 
     python -m par_word2vec_amd.profiling stats  <kernel_stats.csv>
-    python -m par_word2vec_amd.profiling traffic <fetch counter csv> <write counter csv> <copy MiB> <out.json> [workload/write-back]
+    python -m par_word2vec_amd.profiling traffic <fetch counter csv> <write counter csv> <copy MiB> <out.json> [workload/write-back/chunk-words]
 
 `stats` prints the per-kernel dispatch count and average duration of a
 `rocprofv3 --kernel-trace --stats --output-format csv` run.  `traffic` takes
@@ -10,9 +10,9 @@ two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (KiB per dispatch) of a
 command that first copies `copy MiB` device to device (bench.py
 --calibrate-copy-mb), scales each counter by what it reads for that copy
 (bytes per counted KiB) and writes bytes per launch for every training-kernel
-variant, with a tag naming the workload and write-back (default
-"1b/store_out") and a digest of the kernel's sources, so a profile of another
-workload, write-back or kernel build is not used."""
+variant, with a tag naming the workload, write-back and chunk size (default
+"1b/store_out/10000000") and a digest of the kernel's sources, so a profile of
+another workload, write-back, chunking or kernel build is not used."""
 from __future__ import annotations
 
 import csv
@@ -41,10 +41,13 @@ def kernel_source_digest() -> str:
 def variant_of(kernel_name: str) -> str | None:
     """'bf16-persistent' style name of a training-kernel instance, else None
     (mode is not in the symbol; plain and persistent share an instance)."""
-    if "sgns_fast_kernel" not in kernel_name:
+    if "sgns_fast_kernel<" not in kernel_name:
         return None
-    prec = "bf16" if "<unsigned short" in kernel_name else "fp32"
-    rep = "matrix" if kernel_name.rstrip(")").rstrip().endswith("true>") or ", true>" in kernel_name else "vector"
+    # template arguments: element type, pairs per lane, negatives per block,
+    # matrix-core repulsion, W_in and W_out write-back
+    targs = kernel_name.split("sgns_fast_kernel<", 1)[1].split(">", 1)[0].split(",")
+    prec = "bf16" if targs[0].strip() == "unsigned short" else "fp32"
+    rep = "matrix" if len(targs) > 3 and targs[3].strip() == "true" else "vector"
     return f"{prec}-{rep}"
 
 
@@ -57,32 +60,34 @@ def kernel_stats(path: str) -> list[dict]:
     return rows
 
 
-def counter_means(path: str, counter: str) -> dict[str, tuple[int, float]]:
+def counter_values(path: str, counter: str) -> dict[str, list[float]]:
+    """Per kernel name, the counter's value of each dispatch."""
     acc: dict[str, list[float]] = defaultdict(list)
     with open(path) as f:
         for r in csv.DictReader(f):
             if r["Counter_Name"] == counter:
                 acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-    return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
+    return dict(acc)
 
 
-def traffic(fetch_csv: str, write_csv: str, copy_mib: float, workload: str = "1b/store_out") -> dict:
+def traffic(fetch_csv: str, write_csv: str, copy_mib: float, workload: str = "1b/store_out/10000000") -> dict:
     copy_bytes = copy_mib * (1 << 20)
     out = {"workload": workload, "kernel_source": kernel_source_digest(), "copy_bytes": copy_bytes, "counters": {},
            "variants": {}}
     per_kernel: dict[str, dict] = defaultdict(dict)
     for name, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
-        means = counter_means(path, name)
-        copy = [v for k, v in means.items() if COPY_KERNEL in k]
+        vals = counter_values(path, name)
+        copy = [x for k, v in vals.items() if COPY_KERNEL in k for x in v]
         if not copy:
             raise SystemExit(f"no {COPY_KERNEL} dispatch in {path}")
-        n_copy, kib_copy = max(copy, key=lambda t: t[1])  # the calibration copy is the largest
+        kib_copy = max(copy)  # the calibration copy is the largest such dispatch
         factor = copy_bytes / kib_copy  # bytes per counted KiB
         out["counters"][name] = {"copy_kib": kib_copy, "bytes_per_kib": factor}
-        for k, (n, kib) in means.items():
-            v = variant_of(k)
-            if v:
-                per_kernel[v][name] = {"dispatches": n, "kib": kib, "bytes": kib * factor}
+        for k, v in vals.items():
+            var = variant_of(k)
+            if var:
+                kib = sum(v) / len(v)
+                per_kernel[var][name] = {"dispatches": len(v), "kib": kib, "bytes": kib * factor}
     for v, c in per_kernel.items():
         if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
             out["variants"][v] = {"fetch_bytes": c["FETCH_SIZE"]["bytes"], "write_bytes": c["WRITE_SIZE"]["bytes"],

@@ -153,3 +153,27 @@ def test_sentences_per_wave_floor(ragged, floor):
             assert waves > base_waves
         with pytest.raises(DeviceError):
             dev.train("persistent", 0, n, 0.025, T, min_sentences_per_wave=-1)
+
+
+@pytest.mark.parametrize("floor", [4, 64])
+def test_plain_mode_staleness_bound(ragged, floor):
+    """Plain mode with the optional bound keeps at most ceil(n / floor)
+    sentences in flight: the range runs as ceil(n / part) consecutive launches
+    of part = ceil(n / floor) one-wave blocks (fewer than fit on the device),
+    every kept token once.  Without it (0) the range is one launch."""
+    vocab, stream = ragged
+    counts, T = vocab.counts(), vocab.train_words
+    csr = H.subsample(stream, counts, T, 1e-3, seed=5)
+    n = csr.sent_off.size - 1
+    part = -(-n // floor)
+    print(f"[edges] plain floor {floor}: {n} sentences, {part} per launch", flush=True)
+    for prec in ("fp32", "bf16"):
+        with _dev(vocab, prec, *H.init_net(len(vocab), DIM)) as dev:
+            dev.set_sampler(H.unigram_entries(counts))
+            dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
+            st = dev.train("plain", 0, n, 0.025, T, seed=3, min_sentences_per_wave=floor)
+            assert st["tokens"] == csr.tokens.size and st["sentences"] == n
+            assert st["blocks"] == part and st["waves_per_block"] == 1
+            assert dev.last_kernel_ms()[1] == -(-n // part)
+            st = dev.train("plain", 0, n, 0.025, T, seed=3)
+            assert st["tokens"] == csr.tokens.size and st["blocks"] == n and dev.last_kernel_ms()[1] == 1
