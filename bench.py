@@ -265,14 +265,16 @@ def cpu_threads() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(c, seconds: float) -> dict:
+def cpu_baseline(c, seconds: float, windows=None) -> dict:
     """The oracle's Hogwild restatement of PAR_Word2Vec_cpu.cpp run the way
     the reference's harness runs it (cpu:395-416, 702-706): the sample is
     written as a text file and a "word count" vocabulary file, ReadVocab
     reads the vocabulary, each thread seeks to its byte offset, loads its
     share of words and trains it, and the time covers loading and training.
     The vocabulary keeps the whole corpus's counts, so subsampling and the
-    learning-rate schedule are the full corpus's."""
+    learning-rate schedule are the full corpus's.  When the sample is the
+    whole corpus, its tables are scored like the GPU variants' (`windows`:
+    the held-out windows of a Zipf workload)."""
     import shutil
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
@@ -303,6 +305,11 @@ def cpu_baseline(c, seconds: float) -> dict:
         v = O.OracleVocab.read(vpath, text, w.min_count, flavour="fast")
         r = O.train_hogwild_file(v, text, n, v.train_words, w.dim, w.window, w.negative, w.batch_size, 1,
                                  w.alpha, w.sample, threads)
+        scores = None
+        if n == c.stream.size:  # rows in the oracle vocabulary's order -> the corpus vocabulary's
+            rows = np.array([O.lib().orc_vocab_search(v.h, wd.encode()) for wd in words], np.int64)
+            if (rows >= 0).all():
+                scores = quality(c, r["w_in"][rows], r["w_out"][rows], windows)
     finally:
         shutil.rmtree(d, ignore_errors=True)
     st, secs = r["stats"], r["seconds"]
@@ -312,7 +319,7 @@ def cpu_baseline(c, seconds: float) -> dict:
                       f"{r['load_max']:.1f} s); OpenMP Hogwild restatement of PAR_Word2Vec_cpu.cpp with naive "
                       f"GEMM loops in place of MKL",
             "load_max_s": round(r["load_max"], 3), "seconds": round(secs, 3),
-            "host": host_cpu()}
+            "host": host_cpu(), "quality": scores}
 
 
 def main() -> int:
@@ -376,8 +383,8 @@ def main() -> int:
             r["counter_bytes_per_launch"] = t
             r["counter_gbs"] = t / (r["avg_launch_ms"] / 1e3) / 1e9
     scores = {}
+    windows = None
     if not args.no_score:
-        windows = None
         if w.kind == "zipf":
             from par_word2vec_amd import evaluate as E
             held = corpora.heldout_stream(c, 3_000_000, seed=args.seed + 98)
@@ -388,7 +395,7 @@ def main() -> int:
             log(f"{v}: {scores[v]}")
     head = results.get("bf16-persistent") or next(iter(results.values()))
     base = results.get("fp32-plain")
-    cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_seconds)
+    cpu = None if args.no_cpu else cpu_baseline(c, args.cpu_seconds, None if args.no_score else windows)
     line = {
         "metric": "Mwords/sec trained, BF16 persistent vs FP32 plain, at equal planted-cluster score",
         "value": round(head["mwords_s"], 2),
