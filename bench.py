@@ -67,7 +67,7 @@ def algorithmic_bytes(tokens: int, negatives: int, dim: int, elem: int) -> float
 
 def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int, steps: int, csr, seed: int,
                 profile_only: bool = False, repulsion: str = "auto", workload: str = "1b",
-                tune: bool = False, write_back: str = "atomic") -> dict:
+                tune: bool = False, write_back: str = "atomic", min_sentences_per_wave: int = 0) -> dict:
     w = c.workload
     V = len(c.vocab)
     dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision)
@@ -99,7 +99,8 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     def step(i: int, prepare_only: bool = False) -> dict:
         b, e = chunks[i % n]
         return dev.train(mode, b, e, w.alpha, c.train_words, iters=w.iters, epoch=0, seed=seed, use_graph=use_graph,
-                         prepare_only=prepare_only, repulsion=repulsion, write_back=write_back)
+                         prepare_only=prepare_only, repulsion=repulsion, write_back=write_back,
+                         min_sentences_per_wave=min_sentences_per_wave)
 
     for i in range(n):  # queue orders and graphs are built before anything is timed
         step(i, prepare_only=True)
@@ -136,6 +137,7 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
                kernel_mwords_s=raw / (kern_ms / 1e3) / 1e6, alg_bytes=byts, launches=len(stats),
                avg_launch_ms=kern_ms / max(len(stats), 1), achieved_gbs=byts / (kern_ms / 1e3) / 1e9)
     out["grid"] = {k: shape.get(k) for k in ("blocks", "waves_per_block", "sentences_per_dequeue")}
+    out["min_sentences_per_wave"] = min_sentences_per_wave
     if mode == "persistent":
         out["launch"] = launch or {"source": "built-in default (most resident waves, one sentence per dequeue)"}
     if not profile_only:
@@ -334,6 +336,9 @@ def main() -> int:
     ap.add_argument("--write-back", default="store_out", choices=["atomic", "store_out", "store_in", "store"],
                     help="row write-back of both variants: W_in atomics and W_out plain stores (default), all "
                          "atomics, or plain stores of W_in / both tables (bf16 matrix-core path only)")
+    ap.add_argument("--min-sentences-per-wave", type=int, default=0,
+                    help="Hogwild staleness bound of both variants: persistent grid floor (0 = 16 sentences per "
+                         "wave), plain mode at most n / this sentences in flight (0 = one launch)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-score", action="store_true")
@@ -376,7 +381,8 @@ def main() -> int:
                                  profile_only=args.no_score, repulsion=rep[0] if rep else "auto",
                                  workload=args.workload, tune=args.autotune,
                                  write_back=args.write_back if (prec == "bf16" and not rep) or
-                                 args.write_back in ("atomic", "store_out") else "atomic")
+                                 args.write_back in ("atomic", "store_out") else "atomic",
+                                 min_sentences_per_wave=args.min_sentences_per_wave)
     for r in results.values():  # counter-measured HBM bytes (the default chunking's profile)
         t = measured_traffic(args.workload, r["write_back"], args.chunk_words, r["kernel"]) if args.scale == 1.0 else None
         if t is not None:
