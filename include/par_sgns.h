@@ -236,7 +236,7 @@ int par_sgns_set_launch_shape(par_sgns_handle h, int32_t waves_per_block, int32_
  * fetched with par_sgns_get_grads.  flags: PAR_SGNS_WIN_FP64 evaluates the
  * logits and the loss in FP64 (gradient checks); PAR_SGNS_WIN_MFMA runs the
  * (nctx x dim)(dim x (1+nneg)) contraction on BF16 matrix cores with FP32
- * accumulation (BF16 tables only). */
+ * accumulation (BF16 tables below 4 GiB - 4 KiB each). */
 enum { PAR_SGNS_WIN_MFMA = 1, PAR_SGNS_WIN_FP64 = 2 };
 int par_sgns_loss_forward(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
                           int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out);

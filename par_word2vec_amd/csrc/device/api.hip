@@ -859,6 +859,9 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
     if ((flags & PAR_SGNS_WIN_MFMA) && !h->bf16) return seterr(h, PAR_SGNS_EINVAL, "MFMA path needs bf16 tables");
     if ((flags & PAR_SGNS_WIN_MFMA) && (flags & PAR_SGNS_WIN_FP64))
         return seterr(h, PAR_SGNS_EINVAL, "MFMA path accumulates in fp32 (no FP64 variant)");
+    if ((flags & PAR_SGNS_WIN_MFMA) && (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u > 0xFFFFF000ull)
+        return seterr(h, PAR_SGNS_EINVAL, "MFMA path addresses each table through a buffer resource: tables below "
+                      "4 GiB - 4 KiB");
     const int32_t V = h->cfg.vocab_size;
     int rc;
     if ((rc = check_ids(h, centre, nw, 0, V, "centre"))) return rc;
@@ -885,9 +888,11 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
         dc = h->win_buf;
     }
     int32_t *dx = dc + nw, *dn = dx + nw * nctx;
-    HIPCHK(h, hipMemcpyAsync(dc, centre, (size_t)nw * 4, hipMemcpyHostToDevice, h->st));
-    HIPCHK(h, hipMemcpyAsync(dx, ctx, (size_t)nw * nctx * 4, hipMemcpyHostToDevice, h->st));
-    if (nneg > 0) HIPCHK(h, hipMemcpyAsync(dn, neg, (size_t)nw * nneg * 4, hipMemcpyHostToDevice, h->st));
+    if (nw > 0) {  // (an empty batch copies nothing and launches no window kernel)
+        HIPCHK(h, hipMemcpyAsync(dc, centre, (size_t)nw * 4, hipMemcpyHostToDevice, h->st));
+        HIPCHK(h, hipMemcpyAsync(dx, ctx, (size_t)nw * nctx * 4, hipMemcpyHostToDevice, h->st));
+        if (nneg > 0) HIPCHK(h, hipMemcpyAsync(dn, neg, (size_t)nw * nneg * 4, hipMemcpyHostToDevice, h->st));
+    }
     const size_t rows = (size_t)V * (size_t)h->ld;
     if (adjoint) {
         if (!h->g_in) {
@@ -902,10 +907,10 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
     else HIPCHK(h, sgns::launch_reset(h->counters, sgns::kCntSlots, h->loss, nullptr, h->st));
     sgns::WindowArgs w{};
     w.centre = dc; w.ctx = dx; w.neg = dn; w.n_windows = nw; w.nctx = nctx; w.nneg = nneg;
-    w.d = h->cfg.dim; w.ld = h->ld; w.w_in = h->w_in; w.w_out = h->w_out; w.g_in = h->g_in; w.g_out = h->g_out;
+    w.d = h->cfg.dim; w.ld = h->ld; w.V = V; w.w_in = h->w_in; w.w_out = h->w_out; w.g_in = h->g_in; w.g_out = h->g_out;
     w.loss = dloss;
     HIPCHK(h, hipEventRecord(h->ev0, h->st));
-    hipError_t e = sgns::launch_window(w, h->bf16, h->np, adjoint, flags, h->st);
+    hipError_t e = nw > 0 ? sgns::launch_window(w, h->bf16, h->np, adjoint, flags, h->st) : hipSuccess;
     if (e == hipErrorNotSupported) return seterr(h, PAR_SGNS_EINVAL, "requested window path is not built");
     HIPCHK(h, e);
     HIPCHK(h, hipEventRecord(h->ev1, h->st));

@@ -67,7 +67,7 @@ struct FastArgs {
 struct WindowArgs {
     const int32_t *centre, *ctx, *neg;
     int64_t n_windows;
-    int32_t nctx, nneg, d, ld;
+    int32_t nctx, nneg, d, ld, V;
     const void *w_in, *w_out;
     float *g_in, *g_out;          // adjoint outputs (dense, [V][ld])
     double *loss;

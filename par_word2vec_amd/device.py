@@ -155,8 +155,8 @@ class SgnsDevice:
         x = np.ascontiguousarray(ctx, dtype=np.int32)
         n = np.ascontiguousarray(neg, dtype=np.int32)
         nw = c.size
-        nctx = x.size // max(nw, 1)
-        nneg = n.size // max(nw, 1)
+        nctx = x.shape[1] if x.ndim == 2 else x.size // max(nw, 1)
+        nneg = n.shape[1] if n.ndim == 2 else n.size // max(nw, 1)
         out = C.c_double(0.0)
         self._chk(fn(self._h, N.ptr(c), N.ptr(x), N.ptr(n), nw, nctx, nneg, flags, C.byref(out)), fn.__name__)
         return out.value

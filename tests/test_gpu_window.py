@@ -198,3 +198,52 @@ def test_gradient_error_per_batch(windows):
           flush=True)
     assert errs["fp32"][0] < 1e-5
     assert errs["bf16"][0] < 1e-2 and errs["bf16-matrix"][0] < 1e-2
+
+
+@pytest.mark.parametrize("nctx,nneg,dim,vocab", [
+    (1, 0, 64, 5000),     # one-tile kernel, two rows, 2 steps of 32 dimensions
+    (7, 7, 130, 5000),    # one-tile kernel, 15 rows, odd padding
+    (15, 0, 500, 5000),   # one-tile kernel, 16 rows, 16 steps
+    (10, 5, 300, 40),     # one-tile kernel, rows repeated inside windows (atomics add them all)
+    (12, 7, 300, 5000),   # 20 rows: one context tile + the Y tile
+    (24, 3, 200, 5000),   # two context tiles
+    (32, 7, 512, 5000),   # largest window and row: two context tiles, 16 steps, two waves per block
+])
+def test_matrix_window_shapes(nctx, nneg, dim, vocab):
+    """Both matrix-core window kernels (one 16-row tile when 1 + nctx + nneg
+    <= 16, else context tiles + a Y tile) against the FP64 oracle on the same
+    bf16 tables, at the matrix path's bars."""
+    rng = np.random.default_rng(nctx * 100 + nneg)
+    nw = 4096
+    w_in = bf16_round(rng.uniform(-0.5, 0.5, (vocab, dim)).astype(np.float32) / 16)
+    w_out = bf16_round(rng.uniform(-0.5, 0.5, (vocab, dim)).astype(np.float32) / 16)
+    centre = rng.integers(0, vocab, nw).astype(np.int32)
+    ctx = rng.integers(0, vocab, (nw, nctx)).astype(np.int32)
+    neg = rng.integers(0, vocab, (nw, nneg)).astype(np.int32)
+    ref = O.window_loss_f64(centre, ctx, neg, w_in, w_out)
+    gi_ref, go_ref = O.window_grad_f64(centre, ctx, neg, w_in, w_out)
+    with SgnsDevice(vocab, dim, 5, max(nneg, 1), 24, "bf16") as dev:
+        dev.set_weights(w_in, w_out)
+        lf = dev.loss_forward(centre, ctx, neg, flags=WIN_MFMA)
+        la = dev.loss_adjoint(centre, ctx, neg, flags=WIN_MFMA)
+        gi, go = dev.get_grads()
+    print(f"[matrix {nctx}x{nneg} d{dim} V{vocab}] loss {lf:.9f} / {la:.9f} oracle {ref:.9f}; "
+          f"max|dgi| {np.abs(gi - gi_ref).max():.3e}, max|dgo| {np.abs(go - go_ref).max():.3e}", flush=True)
+    assert lf == pytest.approx(ref, rel=1e-6)
+    assert la == pytest.approx(ref, rel=1e-6)
+    assert np.abs(gi - gi_ref).max() <= 1e-4 * np.abs(gi_ref).max()
+    assert np.abs(go - go_ref).max() <= 1e-4 * np.abs(go_ref).max()
+
+
+@pytest.mark.parametrize("flags", [0, WIN_MFMA])
+def test_empty_window_batch(flags):
+    """No windows: zero loss and zero gradients on every path."""
+    with SgnsDevice(1000, 300, 5, 5, 24, "bf16") as dev:
+        rng = np.random.default_rng(5)
+        dev.set_weights(rng.uniform(-0.1, 0.1, (1000, 300)).astype(np.float32),
+                        rng.uniform(-0.1, 0.1, (1000, 300)).astype(np.float32))
+        e = np.zeros(0, np.int32)
+        assert dev.loss_forward(e, e.reshape(0, 10), e.reshape(0, 5), flags=flags) == 0.0
+        assert dev.loss_adjoint(e, e.reshape(0, 10), e.reshape(0, 5), flags=flags) == 0.0
+        gi, go = dev.get_grads()
+        assert not gi.any() and not go.any()
