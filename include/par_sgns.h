@@ -235,8 +235,10 @@ int par_sgns_set_launch_shape(par_sgns_handle h, int32_t waves_per_block, int32_
  * dL/dW_out (FP32, [V][dim], zeroed first) to the handle's gradient buffers,
  * fetched with par_sgns_get_grads.  flags: PAR_SGNS_WIN_FP64 evaluates the
  * logits and the loss in FP64 (gradient checks); PAR_SGNS_WIN_MFMA runs the
- * (nctx x dim)(dim x (1+nneg)) contraction on BF16 matrix cores with FP32
- * accumulation (BF16 tables below 4 GiB - 4 KiB each). */
+ * (nctx x dim)(dim x (1+nneg)) contraction on the matrix cores with FP32
+ * accumulation: BF16 matrix cores on BF16 tables (below 4 GiB - 4 KiB each,
+ * any window), FP32 matrix cores on FP32 tables (windows of at most 16 rows,
+ * 1 + nctx + nneg <= 16). */
 enum { PAR_SGNS_WIN_MFMA = 1, PAR_SGNS_WIN_FP64 = 2 };
 int par_sgns_loss_forward(par_sgns_handle h, const int32_t *centre, const int32_t *ctx, const int32_t *neg,
                           int64_t n_windows, int32_t nctx, int32_t nneg, int32_t flags, double *loss_out);

@@ -856,10 +856,12 @@ static int window_call(par_sgns_handle h, const int32_t *centre, const int32_t *
     if (!centre || !ctx || (nneg > 0 && !neg) || nw < 0 || nctx < 1 || nctx > 32 || nneg < 0 || nneg > 7)
         return seterr(h, PAR_SGNS_EINVAL, "bad window batch (nctx 1..32, nneg 0..7)");
     if (!h->have_weights) return seterr(h, PAR_SGNS_ESTATE, "weights not set");
-    if ((flags & PAR_SGNS_WIN_MFMA) && !h->bf16) return seterr(h, PAR_SGNS_EINVAL, "MFMA path needs bf16 tables");
+    if ((flags & PAR_SGNS_WIN_MFMA) && !h->bf16 && nctx + 1 + nneg > 16)
+        return seterr(h, PAR_SGNS_EINVAL, "MFMA path on fp32 tables takes windows of at most 16 rows "
+                      "(1 + nctx + nneg <= 16); bf16 tables take any window");
     if ((flags & PAR_SGNS_WIN_MFMA) && (flags & PAR_SGNS_WIN_FP64))
         return seterr(h, PAR_SGNS_EINVAL, "MFMA path accumulates in fp32 (no FP64 variant)");
-    if ((flags & PAR_SGNS_WIN_MFMA) && (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u > 0xFFFFF000ull)
+    if ((flags & PAR_SGNS_WIN_MFMA) && h->bf16 && (uint64_t)h->cfg.vocab_size * (uint64_t)h->ld * 2u > 0xFFFFF000ull)
         return seterr(h, PAR_SGNS_EINVAL, "MFMA path addresses each table through a buffer resource: tables below "
                       "4 GiB - 4 KiB");
     const int32_t V = h->cfg.vocab_size;

@@ -405,19 +405,25 @@ __global__ __launch_bounds__(256) void sgns_window_tile_kernel(WindowArgs a, int
     double loss = 0.0;
     int32_t id = id_of(wave);
     for (int64_t w = wave; w < nw; w += nwaves) {
+        // the next window's row id first (loads retire in order: it is then
+        // in hand when this window's rows are), then this window's rows
+        const int32_t id_next = id_of(w + nwaves);
         u32x4_t tv[KS];
         {
             const u32x4_t *p = reinterpret_cast<const u32x4_t *>(tab + (int64_t)id * ld + 8 * g4);
 #pragma unroll
             for (int ks = 0; ks < KS; ks++) tv[ks] = p[4 * ks];
         }
+        // every load issued before the first product: otherwise the scheduler
+        // may interleave them, two registers sets, one round trip per step
+        __builtin_amdgcn_sched_barrier(0);
         // (adjoint) the table and row of the gradient rows this lane adds: 4 g4 + r
         int32_t orow[4];
         if constexpr (ADJ) {
 #pragma unroll
             for (int r = 0; r < 4; r++) orow[r] = __shfl(id, 4 * g4 + r, kWave);
         }
-        id = id_of(w + nwaves);
+        id = id_next;
         f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
@@ -515,6 +521,154 @@ hipError_t launch_tile(const WindowArgs &a, hipStream_t st) {
     return hipErrorNotSupported;
 }
 
+// The one-tile kernel on fp32 tables, on the fp32 matrix cores
+// (v_mfma_f32_16x16x4_f32: A is 16 x 4, lane (r, g) holding A[r][g]; B is
+// 4 x 16, lane (r, g) holding B[g][r]).  Lane (r, g) loads row r of T in
+// 16-byte pieces: piece kb covers dimensions 16 kb + 4 g .. + 3.  A k-step
+// may take the dimensions in any order as long as A and B agree, and in
+// F = T T^T they are the same registers, so component c of piece kb feeds
+// step (kb, c) with k = 16 kb + 4 g + c: four steps per piece, KB = ld / 16
+// pieces.  The adjoint S T takes its k-steps over tile rows the same way:
+// step kk pairs lane group g with tile row 4 g + kk, whose S entry this lane
+// holds in accumulator register kk (S is symmetric), and whose T entry it
+// reads from LDS.  The gradient stays in fp32 (no hi/lo split).
+template <int KB, bool ADJ>
+__global__ __launch_bounds__(256) void sgns_window_tile_f32_kernel(WindowArgs a, int lds_per_wave) {
+    extern __shared__ __align__(16) char smem[];
+    const int lane = threadIdx.x % kWave, r16 = lane & 15, g4 = lane >> 4;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    const int64_t nw = a.n_windows;
+    const int ld = a.ld, d = a.d, SC = win_mfma_stride(ld), JT = (d + 15) / 16;
+    const int nctx = a.nctx, nneg = a.nneg, nr = nctx + 1 + nneg;
+    if (nw <= 0) return;
+    const float *w_in = static_cast<const float *>(a.w_in);
+    const float *w_out = static_cast<const float *>(a.w_out);
+    float *ts = reinterpret_cast<float *>(smem + (size_t)(threadIdx.x / kWave) * lds_per_wave);
+    if constexpr (ADJ) {
+        for (int q = lane; q < ld; q += kWave) ts[nr * SC + q] = 0.f;
+    }
+    const bool is_y = r16 >= nctx && r16 < nr;
+    const int k = r16 - nctx;
+    auto id_of = [&](int64_t w) -> int32_t {
+        const int64_t wc = w < nw ? w : nw - 1;
+        const int32_t *p = !is_y ? a.ctx + wc * nctx + (r16 < nctx ? r16 : 0)
+                                 : (k == 0 ? a.centre + wc : a.neg + wc * nneg + (k - 1));
+        return *p;
+    };
+    const float *tab = is_y ? w_out : w_in;
+    double loss = 0.0;
+    int32_t id = id_of(wave);
+    for (int64_t w = wave; w < nw; w += nwaves) {
+        const int32_t id_next = id_of(w + nwaves);  // (first: see the bf16 kernel)
+        f32x4_t tv[KB];
+        {
+            const f32x4_t *p = reinterpret_cast<const f32x4_t *>(tab + (int64_t)id * ld + 4 * g4);
+#pragma unroll
+            for (int kb = 0; kb < KB; kb++) tv[kb] = p[4 * kb];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // (every load before the first product)
+        int32_t orow[4];
+        if constexpr (ADJ) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) orow[r] = __shfl(id, 4 * g4 + r, kWave);
+        }
+        id = id_next;
+        f32x4_t acc = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(tv[kb][c], tv[kb][c], acc, 0, 0, 0);
+        }
+        if constexpr (ADJ) {
+            if (r16 < nr) {
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++) *reinterpret_cast<f32x4_t *>(ts + r16 * SC + 16 * kb + 4 * g4) = tv[kb];
+            }
+        }
+        float sg4[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 4 * g4 + r;
+            const bool xy = i < nctx && is_y;
+            const bool yx = r16 < nctx && i >= nctx && i < nr;
+            const int yc = xy ? k : i - nctx;
+            const float f = acc[r];
+            if (xy) loss += (double)softplus_stable(yc == 0 ? -f : f);
+            float g = 0.f;
+            if (ADJ && (xy || yx)) {
+                const float sg = 1.0f / (1.0f + expf(-f));
+                g = yc == 0 ? sg - 1.0f : sg;
+            }
+            sg4[r] = g;
+        }
+        if constexpr (ADJ) {
+            float *dst[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 4 * g4 + r;
+                dst[r] = i < nr ? (i < nctx ? a.g_in : a.g_out) + (int64_t)orow[r] * ld : nullptr;
+            }
+            for (int jt = 0; jt < JT; jt++) {
+                f32x4_t o = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int i = 4 * g4 + kk;
+                    const float t = ts[(i < nr ? i : nr) * SC + jt * 16 + r16];
+                    o = __builtin_amdgcn_mfma_f32_16x16x4f32(sg4[kk], t, o, 0, 0, 0);
+                }
+                const int j = jt * 16 + r16;
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (dst[r] && j < d && o[r] != 0.f) unsafeAtomicAdd(dst[r] + j, o[r]);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) loss += __shfl_xor(loss, m, kWave);
+    if (lane == 0 && loss != 0.0) atomicAdd(a.loss, loss);
+}
+
+template <int KB, bool ADJ>
+hipError_t launch_tile_f32_kb(const WindowArgs &a, hipStream_t st) {
+    if (a.n_windows <= 0) return hipSuccess;
+    const int nr = a.nctx + 1 + a.nneg;
+    const int lds = ADJ ? ((nr + 1) * win_mfma_stride(a.ld) * 4 + 15) & ~15 : 0;
+    const int wpb = 4;  // at most 4 x 17 rows of 516 floats: 140 KB
+    auto kern = sgns_window_tile_f32_kernel<KB, ADJ>;
+    if (ADJ) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, lds * wpb);
+        if (e != hipSuccess) return e;
+    }
+    int64_t blocks = (a.n_windows + wpb - 1) / wpb;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kWave * wpb), (size_t)lds * wpb, st, a, lds);
+    return hipGetLastError();
+}
+
+template <bool ADJ>
+hipError_t launch_tile_f32(const WindowArgs &a, hipStream_t st) {
+    switch (a.ld / 32) {  // ld is a multiple of 32 on fp32 tables, at most 512
+    case 1: return launch_tile_f32_kb<2, ADJ>(a, st);
+    case 2: return launch_tile_f32_kb<4, ADJ>(a, st);
+    case 3: return launch_tile_f32_kb<6, ADJ>(a, st);
+    case 4: return launch_tile_f32_kb<8, ADJ>(a, st);
+    case 5: return launch_tile_f32_kb<10, ADJ>(a, st);
+    case 6: return launch_tile_f32_kb<12, ADJ>(a, st);
+    case 7: return launch_tile_f32_kb<14, ADJ>(a, st);
+    case 8: return launch_tile_f32_kb<16, ADJ>(a, st);
+    case 9: return launch_tile_f32_kb<18, ADJ>(a, st);
+    case 10: return launch_tile_f32_kb<20, ADJ>(a, st);
+    case 11: return launch_tile_f32_kb<22, ADJ>(a, st);
+    case 12: return launch_tile_f32_kb<24, ADJ>(a, st);
+    case 13: return launch_tile_f32_kb<26, ADJ>(a, st);
+    case 14: return launch_tile_f32_kb<28, ADJ>(a, st);
+    case 15: return launch_tile_f32_kb<30, ADJ>(a, st);
+    case 16: return launch_tile_f32_kb<32, ADJ>(a, st);
+    }
+    return hipErrorNotSupported;
+}
+
 // LDS bytes per wave of the matrix-core adjoint (the forward uses none).
 inline int win_mfma_lds(const WindowArgs &a) {
     return (a.nctx + 1 + a.nneg + 1) * win_mfma_stride(a.ld) * 2 + 2 * 16 * kWinTS * 2;
@@ -553,8 +707,12 @@ hipError_t launch_mfma_ks(const WindowArgs &a, hipStream_t st) {
 
 hipError_t launch_window(const WindowArgs &a, int bf16, int np, int adjoint, int flags, hipStream_t st) {
     const int fp64 = (flags & 2) != 0;
+    if ((flags & 1) && !bf16) {  // fp32 matrix cores: windows of at most 16 rows, ld <= 512
+        if (fp64 || a.ld > 512 || a.nctx + 1 + a.nneg > 16) return hipErrorNotSupported;
+        return adjoint ? launch_tile_f32<true>(a, st) : launch_tile_f32<false>(a, st);
+    }
     if (flags & 1) {  // matrix cores: bf16 tables below 4 GiB - 4 KiB, fp32 accumulation, ld <= 512
-        if (!bf16 || fp64 || a.ld > 512 || (uint64_t)a.V * (uint64_t)a.ld * 2u > (uint64_t)kWinNoRow)
+        if (fp64 || a.ld > 512 || (uint64_t)a.V * (uint64_t)a.ld * 2u > (uint64_t)kWinNoRow)
             return hipErrorNotSupported;
         if (a.nctx + 1 + a.nneg <= 16) return adjoint ? launch_tile<true>(a, st) : launch_tile<false>(a, st);
         if (a.nctx <= 16) return adjoint ? launch_mfma_ks<1, true>(a, st) : launch_mfma_ks<1, false>(a, st);

@@ -15,6 +15,8 @@ Bars:
   matrix cores (bf16 tables)   forward within 1e-6 relative (exact bf16 products,
                                fp32 sums); adjoint within 1e-4 of max|grad| (the
                                logit gradient enters the products as bf16 hi + lo)
+  matrix cores (fp32 tables)   forward within 1e-6 relative, adjoint within 1e-5 of
+                               max|grad| (fp32 products and sums throughout)
 """
 import numpy as np
 import pytest
@@ -106,12 +108,36 @@ def test_matrix_forward_and_adjoint(windows):
     assert np.abs(go - go_ref).max() <= 1e-4 * np.abs(go_ref).max()
 
 
+def test_fp32_matrix_forward_and_adjoint(windows):
+    """The one-tile window kernel on fp32 tables (fp32 matrix cores) against
+    the FP64 oracle, at the fp32 vector path's bars."""
+    w_in, w_out, centre, ctx, neg = windows
+    ref = O.window_loss_f64(centre, ctx, neg, w_in, w_out)
+    gi_ref, go_ref = O.window_grad_f64(centre, ctx, neg, w_in, w_out)
+    with SgnsDevice(V, D, 5, NNEG, 24, "fp32") as dev:
+        dev.set_weights(w_in, w_out)
+        lf = dev.loss_forward(centre, ctx, neg, flags=WIN_MFMA)
+        f_ms, _ = dev.last_kernel_ms(1)
+        dev.loss_forward(centre, ctx, neg)
+        fv_ms, _ = dev.last_kernel_ms(1)
+        la = dev.loss_adjoint(centre, ctx, neg, flags=WIN_MFMA)
+        a_ms, _ = dev.last_kernel_ms(2)
+        gi, go = dev.get_grads()
+    print(f"[fp32 matrix] forward {f_ms:.3f} ms (vector {fv_ms:.3f}), adjoint {a_ms:.3f} ms; "
+          f"max|dgi| {np.abs(gi - gi_ref).max():.3e} of {np.abs(gi_ref).max():.3e}", flush=True)
+    assert lf == pytest.approx(ref, rel=1e-6)
+    assert la == pytest.approx(ref, rel=1e-6)
+    assert np.abs(gi - gi_ref).max() <= 1e-5 * np.abs(gi_ref).max()
+    assert np.abs(go - go_ref).max() <= 1e-5 * np.abs(go_ref).max()
+
+
 def test_matrix_window_errors(windows):
     w_in, w_out, centre, ctx, neg = windows
     with SgnsDevice(V, D, 5, NNEG, 24, "fp32") as dev:
         dev.set_weights(w_in, w_out)
-        with pytest.raises(DeviceError, match="bf16"):
-            dev.loss_forward(centre[:8], ctx[:8], neg[:8], flags=WIN_MFMA)
+        wide = np.concatenate([ctx[:8], ctx[:8]], axis=1)  # 1 + 20 + 5 rows
+        with pytest.raises(DeviceError, match="16 rows"):
+            dev.loss_forward(centre[:8], wide, neg[:8], flags=WIN_MFMA)
     with SgnsDevice(V, D, 5, NNEG, 24, "bf16") as dev:
         dev.set_weights(w_in, w_out)
         with pytest.raises(DeviceError, match="fp32"):
@@ -212,7 +238,8 @@ def test_gradient_error_per_batch(windows):
 def test_matrix_window_shapes(nctx, nneg, dim, vocab):
     """Both matrix-core window kernels (one 16-row tile when 1 + nctx + nneg
     <= 16, else context tiles + a Y tile) against the FP64 oracle on the same
-    bf16 tables, at the matrix path's bars."""
+    bf16 tables, at the matrix path's bars; windows of at most 16 rows also on
+    fp32 tables (fp32 matrix cores) at the fp32 bars."""
     rng = np.random.default_rng(nctx * 100 + nneg)
     nw = 4096
     w_in = bf16_round(rng.uniform(-0.5, 0.5, (vocab, dim)).astype(np.float32) / 16)
@@ -233,12 +260,26 @@ def test_matrix_window_shapes(nctx, nneg, dim, vocab):
     assert la == pytest.approx(ref, rel=1e-6)
     assert np.abs(gi - gi_ref).max() <= 1e-4 * np.abs(gi_ref).max()
     assert np.abs(go - go_ref).max() <= 1e-4 * np.abs(go_ref).max()
+    if 1 + nctx + nneg > 16:
+        return
+    with SgnsDevice(vocab, dim, 5, max(nneg, 1), 24, "fp32") as dev:
+        dev.set_weights(w_in, w_out)
+        lf = dev.loss_forward(centre, ctx, neg, flags=WIN_MFMA)
+        la = dev.loss_adjoint(centre, ctx, neg, flags=WIN_MFMA)
+        gi, go = dev.get_grads()
+    print(f"[fp32 matrix {nctx}x{nneg} d{dim} V{vocab}] loss {lf:.9f} / {la:.9f}; "
+          f"max|dgi| {np.abs(gi - gi_ref).max():.3e}, max|dgo| {np.abs(go - go_ref).max():.3e}", flush=True)
+    assert lf == pytest.approx(ref, rel=1e-6)
+    assert la == pytest.approx(ref, rel=1e-6)
+    assert np.abs(gi - gi_ref).max() <= 1e-5 * np.abs(gi_ref).max()
+    assert np.abs(go - go_ref).max() <= 1e-5 * np.abs(go_ref).max()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
 @pytest.mark.parametrize("flags", [0, WIN_MFMA])
-def test_empty_window_batch(flags):
+def test_empty_window_batch(flags, precision):
     """No windows: zero loss and zero gradients on every path."""
-    with SgnsDevice(1000, 300, 5, 5, 24, "bf16") as dev:
+    with SgnsDevice(1000, 300, 5, 5, 24, precision) as dev:
         rng = np.random.default_rng(5)
         dev.set_weights(rng.uniform(-0.1, 0.1, (1000, 300)).astype(np.float32),
                         rng.uniform(-0.1, 0.1, (1000, 300)).astype(np.float32))
