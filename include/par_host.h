@@ -157,6 +157,11 @@ int64_t ph_synth_zipf(int64_t n_tokens, int32_t raw_vocab, double zipf_s, int32_
 int64_t ph_synth_sbm_walks(int32_t n_nodes, int32_t n_comm, double exponent, double mean_degree,
                            double p_intra, int32_t walks_per_node, int32_t walk_len, uint64_t seed,
                            int32_t *out, int64_t cap, int32_t *comm);
+/* The graph behind ph_synth_sbm_walks with the same arguments and seed:
+ * degree[n_nodes] receives each node's degree, *intra_edges the number of
+ * edges inside a community.  Returns the number of edges. */
+int64_t ph_synth_sbm_graph(int32_t n_nodes, int32_t n_comm, double exponent, double mean_degree,
+                           double p_intra, uint64_t seed, int64_t *degree, int64_t *intra_edges);
 /* Writes the raw corpus as text: "w<id>" tokens, one line per sentence.
  * Graph corpora use the same rendering. */
 int ph_write_raw_text(const char *path, const int32_t *raw, int64_t n);

@@ -93,6 +93,7 @@ HOST_SYMBOLS = {
     "ph_synth_zipf": (i64, [i64, i32, f64, i32, f64, f64, i32, i32, i32, i32, i32, i32, i32, f64, u64,
                            vp, i64, vp]),
     "ph_synth_sbm_walks": (i64, [i32, i32, f64, f64, f64, i32, i32, u64, vp, i64, vp]),
+    "ph_synth_sbm_graph": (i64, [i32, i32, f64, f64, f64, u64, vp, vp]),
     "ph_write_raw_text": (C.c_int, [cp, vp, i64]),
     "ph_corpus_save": (C.c_int, [cp, vp, vp, i64]),
     "ph_corpus_vocab": (vp, [cp, vp]),

@@ -878,11 +878,16 @@ int64_t ph_synth_zipf(int64_t n_tokens, int32_t raw_vocab, double zipf_s, int32_
     return off[(size_t)nlines];
 }
 
-int64_t ph_synth_sbm_walks(int32_t N, int32_t C, double expo, double mean_deg, double p_intra,
-                           int32_t walks_per_node, int32_t walk_len, uint64_t seed,
-                           int32_t *out, int64_t cap, int32_t *comm) {
-    if (N < 2 || C < 1 || C > N || expo <= 2.0 || mean_deg <= 0 || walk_len < 1 || walks_per_node < 1 || !out)
-        return fail(PH_EINVAL, "bad sbm arguments");
+namespace {
+// The degree-corrected SBM of ph_synth_sbm_walks / ph_synth_sbm_graph as an
+// adjacency list (ro: CSR offsets, adj: both ends of every edge).
+struct SbmGraph {
+    std::vector<int64_t> ro;
+    std::vector<int32_t> adj;
+    int64_t intra = 0;  // edges inside a community
+};
+SbmGraph sbm_build(int32_t N, int32_t C, double expo, double mean_deg, double p_intra, uint64_t seed,
+                   int32_t *comm) {
     Rng r(mix64(seed ^ 0x53424DULL));
     // target degrees: Pareto(expo) scaled to the requested mean
     std::vector<double> deg((size_t)N);
@@ -908,6 +913,7 @@ int64_t ph_synth_sbm_walks(int32_t N, int32_t C, double expo, double mean_deg, d
     int64_t M = (int64_t)std::llround(N * mean_deg / 2.0);
     std::vector<std::pair<int32_t, int32_t>> edges;
     edges.reserve((size_t)M);
+    SbmGraph g;
     while ((int64_t)edges.size() < M) {
         int32_t u = all.sample(r), v;
         int32_t cu = u % C;
@@ -917,13 +923,39 @@ int64_t ph_synth_sbm_walks(int32_t N, int32_t C, double expo, double mean_deg, d
         }
         if (u == v) continue;
         edges.emplace_back(u, v);
+        g.intra += v % C == cu;
     }
-    std::vector<int64_t> ro((size_t)N + 1, 0);
-    for (auto &e : edges) { ro[(size_t)e.first + 1]++; ro[(size_t)e.second + 1]++; }
-    for (int32_t u = 0; u < N; u++) ro[(size_t)u + 1] += ro[(size_t)u];
-    std::vector<int32_t> adj((size_t)ro[(size_t)N]);
-    std::vector<int64_t> fill(ro.begin(), ro.end() - 1);
-    for (auto &e : edges) { adj[(size_t)fill[(size_t)e.first]++] = e.second; adj[(size_t)fill[(size_t)e.second]++] = e.first; }
+    g.ro.assign((size_t)N + 1, 0);
+    for (auto &e : edges) { g.ro[(size_t)e.first + 1]++; g.ro[(size_t)e.second + 1]++; }
+    for (int32_t u = 0; u < N; u++) g.ro[(size_t)u + 1] += g.ro[(size_t)u];
+    g.adj.resize((size_t)g.ro[(size_t)N]);
+    std::vector<int64_t> fill(g.ro.begin(), g.ro.end() - 1);
+    for (auto &e : edges) {
+        g.adj[(size_t)fill[(size_t)e.first]++] = e.second;
+        g.adj[(size_t)fill[(size_t)e.second]++] = e.first;
+    }
+    return g;
+}
+}  // namespace
+
+int64_t ph_synth_sbm_graph(int32_t N, int32_t C, double expo, double mean_deg, double p_intra, uint64_t seed,
+                           int64_t *degree, int64_t *intra_edges) {
+    if (N < 2 || C < 1 || C > N || expo <= 2.0 || mean_deg <= 0 || !degree || !intra_edges)
+        return fail(PH_EINVAL, "bad sbm arguments");
+    SbmGraph g = sbm_build(N, C, expo, mean_deg, p_intra, seed, nullptr);
+    for (int32_t u = 0; u < N; u++) degree[u] = g.ro[(size_t)u + 1] - g.ro[(size_t)u];
+    *intra_edges = g.intra;
+    return g.ro[(size_t)N] / 2;
+}
+
+int64_t ph_synth_sbm_walks(int32_t N, int32_t C, double expo, double mean_deg, double p_intra,
+                           int32_t walks_per_node, int32_t walk_len, uint64_t seed,
+                           int32_t *out, int64_t cap, int32_t *comm) {
+    if (N < 2 || C < 1 || C > N || expo <= 2.0 || mean_deg <= 0 || walk_len < 1 || walks_per_node < 1 || !out)
+        return fail(PH_EINVAL, "bad sbm arguments");
+    SbmGraph g = sbm_build(N, C, expo, mean_deg, p_intra, seed, comm);
+    const std::vector<int64_t> &ro = g.ro;
+    const std::vector<int32_t> &adj = g.adj;
     int64_t nwalks = (int64_t)N * walks_per_node;
     if (nwalks * (walk_len + 1) > cap) return fail(PH_ERANGE, "walk output capacity too small");
     std::vector<int32_t> wl((size_t)nwalks);

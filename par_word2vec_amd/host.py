@@ -230,6 +230,18 @@ def synth_sbm_walks(n_nodes: int = 10_000, n_comm: int = 20, exponent: float = 2
     return out[:n], comm
 
 
+def synth_sbm_graph(n_nodes: int = 10_000, n_comm: int = 20, exponent: float = 2.5, mean_degree: float = 40.0,
+                    p_intra: float = 0.85, seed: int = 3) -> tuple[np.ndarray, int, int]:
+    """(degree per node, edges, edges inside a community) of the graph that
+    synth_sbm_walks walks with the same arguments and seed."""
+    import ctypes as C
+    deg = np.empty(n_nodes, dtype=np.int64)
+    intra = C.c_int64(0)
+    m = _chk(N.host().ph_synth_sbm_graph(n_nodes, n_comm, exponent, mean_degree, p_intra, seed, N.ptr(deg),
+                                         C.byref(intra)), "synth sbm graph")
+    return deg, int(m), intra.value
+
+
 def save_corpus(path: str, vocab: Vocab, stream: np.ndarray) -> None:
     """Binary corpus: vocabulary + id stream in one file (ph_corpus_save)."""
     s = np.ascontiguousarray(stream, dtype=np.int32)

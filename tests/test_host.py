@@ -201,6 +201,32 @@ def test_synthetic_sbm_walks():
     assert nodes.max() < 2000
 
 
+def test_sbm_graph_matches_its_specification():
+    """The config-3 graph (10,000 nodes, 20 communities, degree exponent 2.5,
+    mean degree 40, 85% intra-community edges): the mean degree is 40, the
+    fraction of intra-community edges is 0.85 (less the self-loops the
+    generator redraws), and the Hill estimate of the degree tail over the top
+    5% of nodes is the power law's exponent minus one (1.5; 2.0 would mean
+    exponent 3).  The walks step along intra-community edges at the same rate."""
+    deg, m, intra = H.synth_sbm_graph(n_nodes=10_000, n_comm=20, exponent=2.5, mean_degree=40.0, p_intra=0.85,
+                                      seed=3)
+    assert m == 200_000 and deg.sum() == 2 * m
+    assert deg.mean() == pytest.approx(40.0, rel=1e-3)
+    assert intra / m == pytest.approx(0.85, abs=0.005)
+    d = np.sort(deg)[::-1].astype(np.float64)
+    k = 500
+    hill = k / np.log(d[:k] / d[k]).sum()
+    assert hill == pytest.approx(1.5, abs=0.15)
+    other, _, _ = H.synth_sbm_graph(n_nodes=10_000, n_comm=20, exponent=3.5, mean_degree=40.0, p_intra=0.85, seed=3)
+    o = np.sort(other)[::-1].astype(np.float64)
+    assert k / np.log(o[:k] / o[k]).sum() > 2.0  # a steeper law is told apart
+    raw, comm = H.synth_sbm_walks(n_nodes=10_000, n_comm=20, exponent=2.5, mean_degree=40.0, p_intra=0.85,
+                                  walks_per_node=2, walk_len=80, seed=3)
+    a, b = raw[:-1], raw[1:]
+    mk = (a > 0) & (b > 0)
+    assert (comm[a[mk] - 1] == comm[b[mk] - 1]).mean() == pytest.approx(intra / m, abs=0.01)
+
+
 def test_workloads_generate_at_small_scale():
     for name in ("zipf1m", "sbm"):
         c = corpora.make(name, scale=0.05)
