@@ -362,27 +362,28 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
     auto srow = [&](int i) { return i < ni ? i : a.B; };  // tile rows past the batch read the zero row
     double lr = 0.0;  // this lane's share of the batch loss (summed over the wave at the end)
     {
-        // snapshot of the batch rows (after this sentence's attraction), eight in
-        // flight, and the zero row
+        // snapshot of the batch rows (after this sentence's attraction) and the
+        // zero row.  The rows go from the table straight into LDS (buffer loads
+        // with an LDS destination: lane l's 4 bytes land at the instruction's
+        // LDS address + 4 l), 256 bytes of a row per instruction, every row of
+        // the batch in flight at once and no registers staged; the lanes past
+        // the row are switched off, so nothing lands in the row's padding or
+        // the next row.  One wait for all of them follows (such loads count
+        // in vmcnt like any other).
+        // (the batch's row ids are read first, lane i holding row i: a load
+        // inside the loop would wait for every row already in flight)
         for (int q = lane; 2 * q < ld; q += kWave) *reinterpret_cast<uint32_t *>(snapC + a.B * SC + 2 * q) = 0u;
-        for (int i = 0; i < ni; i += 8) {
-            uint32_t rows[8][NP];
+        const int bat_id = lane < ni ? bat[lane] : 0;  // ni <= kMfI * 16 <= 64
+        for (int i = 0; i < ni; i++) {
+            const uint32_t off = (uint32_t)__builtin_amdgcn_readlane(bat_id, i) * row_bytes + 4u * (uint32_t)lane;
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t off = (i + k < ni ? (uint32_t)bat[i + k] * row_bytes : kNoRow) + 4u * (uint32_t)lane;
-#pragma unroll
-                for (int t = 0; t < NP; t++)  // lanes past the row read the next row's start (not stored)
-                    rows[k][t] = __builtin_amdgcn_raw_buffer_load_b32(rin, off + 256u * t, 0, kSc1);
-            }
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-#pragma unroll
-                for (int t = 0; t < NP; t++) {
-                    const int q = lane + kWave * t;
-                    if (i + k < ni && 2 * q < ld) *reinterpret_cast<uint32_t *>(snapC + (i + k) * SC + 2 * q) = rows[k][t];
-                }
+            for (int t = 0; t < NP; t++) {
+                auto *dst = (__attribute__((address_space(3))) void *)(snapC + i * SC + 128 * t);
+                if (t < NP - 1 || 2 * (lane + kWave * t) < ld)
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rin, dst, 4, off + 256u * t, 0, 0, kSc1);
             }
         }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the snapshot has landed
         int32_t rowC[kMfI];
 #pragma unroll
         for (int it = 0; it < kMfI; it++) rowC[it] = it * 16 + r16 < ni ? bat[it * 16 + r16] : -1;
