@@ -67,10 +67,11 @@ def algorithmic_bytes(tokens: int, negatives: int, dim: int, elem: int) -> float
 
 def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int, steps: int, csr, seed: int,
                 profile_only: bool = False, repulsion: str = "auto", workload: str = "1b",
-                tune: bool = False, write_back: str = "atomic", min_sentences_per_wave: int = 0) -> dict:
+                tune: bool = False, write_back: str = "atomic", min_sentences_per_wave: int = 0,
+                rounding: str = "stochastic") -> dict:
     w = c.workload
     V = len(c.vocab)
-    dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision)
+    dev = SgnsDevice(V, w.dim, w.window, w.negative, w.batch_size, precision, rounding=rounding)
     dev.set_exp_table(H.exp_table())
     w_in, w_out = H.init_net(V, w.dim)
     dev.set_weights(w_in, w_out)
@@ -130,7 +131,8 @@ def run_variant(c, mode: str, precision: str, chunks, raw_per_chunk, warmup: int
     shape = stats[-1] if stats else {}
     # the same rate when the epoch's corpus upload (host buffers across PCIe) is charged pro rata
     pcie_s = wall + upload_s * raw / max(int(c.stream.size), 1)
-    out = dict(mode=mode, precision=precision, kernel=f"{precision}-{kind}", write_back=write_back, wall_s=wall,
+    out = dict(mode=mode, precision=precision, kernel=f"{precision}-{kind}", write_back=write_back,
+               rounding=rounding if precision == "bf16" else None, wall_s=wall,
                upload_s=upload_s,
                pcie_inclusive_mwords_s=raw / pcie_s / 1e6, kernel_ms=kern_ms, raw_words=raw, tokens=tokens,
                negatives=negs, pairs=sum(s["pairs"] for s in stats), mwords_s=raw / wall / 1e6,
@@ -339,6 +341,9 @@ def main() -> int:
     ap.add_argument("--min-sentences-per-wave", type=int, default=0,
                     help="Hogwild staleness bound of both variants: persistent grid floor (0 = 16 sentences per "
                          "wave), plain mode at most n / this sentences in flight (0 = one launch)")
+    ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest", "nearest_out", "nearest_in"],
+                    help="bf16 write-back rounding: stochastic (default) or round-to-nearest for both tables, or "
+                         "nearest for W_out (nearest_out) / W_in (nearest_in) rows only")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-score", action="store_true")
@@ -382,7 +387,7 @@ def main() -> int:
                                  workload=args.workload, tune=args.autotune,
                                  write_back=args.write_back if (prec == "bf16" and not rep) or
                                  args.write_back in ("atomic", "store_out") else "atomic",
-                                 min_sentences_per_wave=args.min_sentences_per_wave)
+                                 min_sentences_per_wave=args.min_sentences_per_wave, rounding=args.rounding)
     for r in results.values():  # counter-measured HBM bytes (the default chunking's profile)
         t = measured_traffic(args.workload, r["write_back"], args.chunk_words, r["kernel"]) if args.scale == 1.0 else None
         if t is not None:

@@ -37,8 +37,15 @@ enum {
 enum par_sgns_precision { PAR_SGNS_FP32 = 0, PAR_SGNS_BF16 = 1 };
 /* How a bf16 table entry absorbs an fp32 update: stochastic rounding (the
  * default; unbiased for updates below half an ulp) or round-to-nearest
- * (ties away from zero). */
-enum par_sgns_rounding { PAR_SGNS_ROUND_STOCHASTIC = 0, PAR_SGNS_ROUND_NEAREST = 1 };
+ * (ties away from zero), for both tables or round-to-nearest for one table
+ * only (NEAREST_OUT: W_out rows nearest, W_in rows stochastic; NEAREST_IN the
+ * reverse).  Training modes only; the deterministic mode is fp32. */
+enum par_sgns_rounding {
+    PAR_SGNS_ROUND_STOCHASTIC = 0,
+    PAR_SGNS_ROUND_NEAREST = 1,
+    PAR_SGNS_ROUND_NEAREST_OUT = 2,
+    PAR_SGNS_ROUND_NEAREST_IN = 3,
+};
 
 /* How plain/persistent modes evaluate a repulsion batch (the three SGEMMs
  * of PAR_Word2Vec_cpu.cpp:633-656).  AUTO: matrix cores on bf16 tables,

@@ -208,7 +208,7 @@ int par_sgns_create(const par_sgns_config *cfg, par_sgns_handle *out) {
     if (cfg->vocab_size < 2 || cfg->dim < 1 || cfg->dim > 512 || cfg->window < 1 || cfg->window > 31 ||
         cfg->negative < 0 || cfg->negative > 64 || cfg->batch_size < 1 || cfg->batch_size > 64 ||
         (cfg->precision != PAR_SGNS_FP32 && cfg->precision != PAR_SGNS_BF16) ||
-        (cfg->rounding != PAR_SGNS_ROUND_STOCHASTIC && cfg->rounding != PAR_SGNS_ROUND_NEAREST))
+        cfg->rounding < PAR_SGNS_ROUND_STOCHASTIC || cfg->rounding > PAR_SGNS_ROUND_NEAREST_IN)
         return PAR_SGNS_EINVAL;
     par_sgns_handle h = new par_sgns_handle_s;
     h->cfg = *cfg;

@@ -45,6 +45,11 @@ namespace fast {
 constexpr int kMaxRedraw = 64;
 constexpr int kUnusedSlot = -(1 << 30);
 
+// Rounding of bf16 write-backs per table (par_sgns_rounding): stochastic for
+// both, round-to-nearest for both, or nearest for one table only.
+__device__ __forceinline__ int nearest_in(const FastArgs &a) { return a.rounding == 1 || a.rounding == 3; }
+__device__ __forceinline__ int nearest_out(const FastArgs &a) { return a.rounding == 1 || a.rounding == 2; }
+
 // ---------------------------------------------------------------- stores
 // Every write-back is a no-return atomic: fp32 tables add the fp32 delta;
 // bf16 tables decide the rounding of base + delta against the wave's own
@@ -554,7 +559,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
                             cY[it] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(gd[ut][it], bY, cY[it], 0, 0, 0);
                         }
                     }
-                    write_pair<WOUT>(rout, ut < UT, offN[ut], base, nX, nY, dimp, odd, rs, a.rounding, a.hot_bytes);
+                    write_pair<WOUT>(rout, ut < UT, offN[ut], base, nX, nY, dimp, odd, rs, nearest_out(a), a.hot_bytes);
                 }
                 // dC write-back, rounding against the snapshot
 #pragma unroll
@@ -563,7 +568,7 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                     for (int r = 0; r < 4; r++)
                         base[r] = *reinterpret_cast<const uint32_t *>(snapC + srow(it * 16 + 4 * g4 + r) * SC + dimp);
-                    write_pair<WIN>(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, odd, rs, a.rounding,
+                    write_pair<WIN>(rin, it < IT, offC[it], base, cX[it], cY[it], dimp, odd, rs, nearest_in(a),
                                     a.hot_bytes);
                 }
             };
@@ -686,12 +691,12 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     };
     auto write_slot = [&](int64_t row, const float2 (&u)[NP], const float2 (&u0)[NP]) {
         if constexpr (MF) {
-            store_value_rsrc<NP, WIN>(rin_w, (uint32_t)row, row_bytes, nq, lane, u0, u, rs, a.rounding, a.hot_bytes);
+            store_value_rsrc<NP, WIN>(rin_w, (uint32_t)row, row_bytes, nq, lane, u0, u, rs, nearest_in(a), a.hot_bytes);
         } else {
             float2 du[NP];
 #pragma unroll
             for (int t = 0; t < NP; t++) du[t] = make_float2(u[t].x - u0[t].x, u[t].y - u0[t].y);
-            store_delta<T, NP>(w_in, row, ld, nq, lane, u0, du, rs, a.rounding);
+            store_delta<T, NP>(w_in, row, ld, nq, lane, u0, du, rs, nearest_in(a));
         }
     };
     auto flush = [&](int slot) {
@@ -912,13 +917,13 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             // this centre's write-backs
             if (!carry_next) {
                 if constexpr (MF) {
-                    store_value_rsrc<NP, WOUT>(rout_w, (uint32_t)wp, row_bytes, nq, lane, v0, v, rs, a.rounding,
+                    store_value_rsrc<NP, WOUT>(rout_w, (uint32_t)wp, row_bytes, nq, lane, v0, v, rs, nearest_out(a),
                                                a.hot_bytes);
                 } else {
                     float2 dv[NP];
 #pragma unroll
                     for (int t = 0; t < NP; t++) dv[t] = make_float2(v[t].x - v0[t].x, v[t].y - v0[t].y);
-                    store_delta<T, NP, WOUT == kWbStore>(w_out, wp, ld, nq, lane, v0, dv, rs, a.rounding);
+                    store_delta<T, NP, WOUT == kWbStore>(w_out, wp, ld, nq, lane, v0, dv, rs, nearest_out(a));
                 }
             }
             if (leave) write_slot(frow, fu, fu0);
@@ -1126,7 +1131,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                         dn[k][t].x *= mult[k];
                         dn[k][t].y *= mult[k];
                     }
-                    store_delta<T, NP, WOUT == kWbStore>(w_out, nrow[k], ld, nq, lane, nv[k], dn[k], rs, a.rounding);
+                    store_delta<T, NP, WOUT == kWbStore>(w_out, nrow[k], ld, nq, lane, nv[k], dn[k], rs, nearest_out(a));
                 }
             }
         }
@@ -1145,7 +1150,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 const int q = lane + kWave * t;
                 dc[t] = t < NP - 1 || q < nq ? acc[i * nq + q] : make_float2(0.0f, 0.0f);
             }
-            store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, c, dc, rs, a.rounding);
+            store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, c, dc, rs, nearest_in(a));
         }
     }
 }

@@ -35,6 +35,7 @@ struct Opts {
     int autotune = 0;  // persistent mode: tune the launch shape on the first chunk, this many runs per shape
     int min_sent_wave = 0;  // staleness bound: persistent grid floor (0 = 16); plain mode: sentences in flight <= n / this (0 = off)
     std::string write_back = "atomic";  // bf16 matrix-core write-back policy
+    std::string rounding = "stochastic";  // bf16 write-back rounding (per table: par_sgns_rounding)
     int hot_rows = 0;                   // write_back hot: rows with id < this keep atomics
 };
 
@@ -50,6 +51,7 @@ void usage() {
         "  -min-sentences-per-wave <int>: staleness bound; persistent mode: sentences per wave (16);\n"
         "    plain mode: at most n / this sentences in flight (0: off)\n"
         "  -write-back atomic|store_out|store_in|store|hot (atomic) -hot-rows <int>: bf16 row write-back\n"
+        "  -rounding stochastic|nearest|nearest_out|nearest_in (stochastic): bf16 write-back rounding\n"
         "  -train-bin <file>: binary corpus (vocabulary + ids) in place of -train / -read-vocab\n"
         "  -save-bin <file>: write the binary corpus of -train (then train only if -output is given)\n"
         "  accepted and ignored: -threads -disk -gamma -max-num-sen\n");
@@ -92,6 +94,7 @@ int main(int argc, char **argv) {
     num("-autotune", o.autotune);
     num("-min-sentences-per-wave", o.min_sent_wave);
     str("-write-back", o.write_back);
+    str("-rounding", o.rounding);
     num("-hot-rows", o.hot_rows);
     str("-time-file", o.time_file);
     str("-train-bin", o.train_bin);
@@ -102,6 +105,11 @@ int main(int argc, char **argv) {
     const int mode = o.mode == "det" ? PAR_SGNS_DETERMINISTIC : o.mode == "plain" ? PAR_SGNS_PLAIN : PAR_SGNS_PERSISTENT;
     if (o.mode != "det" && o.mode != "plain" && o.mode != "persistent") return fail("-mode", "det, plain or persistent");
     if (o.precision != "fp32" && o.precision != "bf16") return fail("-precision", "fp32 or bf16");
+    const char *roundings[] = {"stochastic", "nearest", "nearest_out", "nearest_in"};
+    int rounding = -1;
+    for (int k = 0; k < 4; k++)
+        if (o.rounding == roundings[k]) rounding = k;
+    if (rounding < 0) return fail("-rounding", "stochastic, nearest, nearest_out or nearest_in");
     if (o.train.empty() && o.train_bin.empty()) return fail("-train", "training file (or -train-bin) required");
     if (!o.train.empty() && !o.train_bin.empty()) return fail("-train-bin", "give -train or -train-bin, not both");
     if (!o.train_bin.empty() && !o.save_bin.empty()) return fail("-save-bin", "the corpus is binary already");
@@ -146,7 +154,7 @@ int main(int argc, char **argv) {
 
     // device handle, tables, weights (cpu:387-388; text.cu:348-376, 1092-1100)
     par_sgns_config cfg{V, o.size, o.window, o.negative, o.batch,
-                        o.precision == "bf16" ? PAR_SGNS_BF16 : PAR_SGNS_FP32, o.device, PAR_SGNS_ROUND_STOCHASTIC};
+                        o.precision == "bf16" ? PAR_SGNS_BF16 : PAR_SGNS_FP32, o.device, rounding};
     par_sgns_handle h = nullptr;
     if (par_sgns_create(&cfg, &h) != 0) return fail("device", h ? par_sgns_last_error(h) : "invalid configuration");
     std::vector<float> et(1001);

@@ -14,7 +14,7 @@ DETERMINISTIC, PLAIN, PERSISTENT = 0, 1, 2
 MODES = {"det": DETERMINISTIC, "deterministic": DETERMINISTIC, "plain": PLAIN, "persistent": PERSISTENT}
 PRECISIONS = {"fp32": FP32, "bf16": BF16}
 WIN_MFMA, WIN_FP64 = 1, 2
-ROUNDINGS = {"stochastic": 0, "nearest": 1}
+ROUNDINGS = {"stochastic": 0, "nearest": 1, "nearest_out": 2, "nearest_in": 3}
 # repulsion batch evaluation: matrix cores on bf16 tables unless "vector" is asked for
 REPULSIONS = {"auto": 0, "vector": 1, "matrix": 2}
 # bf16 matrix-core write-back (par_sgns_write_back): atomics, plain stores of W_out rows, of W_in

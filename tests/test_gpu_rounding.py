@@ -96,3 +96,32 @@ def test_stochastic_rounding_is_unbiased(setup, rounding, repulsion, write_back)
         assert abs(mean) < 4 * se
     else:
         assert slope < 0.5
+
+
+@pytest.mark.parametrize("rounding", ["nearest_out", "nearest_in"])
+def test_rounding_per_table(setup, rounding):
+    """Round-to-nearest on one table only reaches that table alone: it
+    stagnates (slope below 0.5; exactly 0 measured, as with nearest on both),
+    while the other keeps stochastic rounding (slope above 0.75; 0.88 and 1.06
+    measured).  Unbiasedness itself is the test above: here the fp32 run is only
+    approximately the stochastic table's reference, since the other table stays
+    at its starting values and the gradients depend on it (an element of W_in
+    drifts by up to a few percent of its value over the pass in fp32)."""
+    vocab, plan, T, (w_in, w_out), (f_in, f_out) = setup
+    with SgnsDevice(len(vocab), DIM, W, NEG, B, "bf16", rounding=rounding) as dev:
+        dev.set_exp_table(H.exp_table())
+        dev.set_weights(w_in, w_out)
+        dev.set_plan(plan)
+        dev.train("persistent", 0, plan.n_sent, ALPHA, T, planned=True, blocks=1, waves_per_block=1,
+                  repulsion="matrix", write_back="atomic")
+        b_in, b_out = dev.get_weights()
+    s_in = _stats((b_in - w_in).ravel().astype(np.float64), (f_in - w_in).ravel().astype(np.float64),
+                  w_in.ravel().astype(np.float64))
+    s_out = _stats((b_out - w_out).ravel().astype(np.float64), (f_out - w_out).ravel().astype(np.float64),
+                   w_out.ravel().astype(np.float64))
+    print(f"[rounding] {rounding}: W_in slope {s_in[0]:.4f} ({s_in[3]} elements), "
+          f"W_out slope {s_out[0]:.4f} ({s_out[3]} elements)", flush=True)
+    sr, rn = (s_in, s_out) if rounding == "nearest_out" else (s_out, s_in)
+    assert sr[3] > 1000 and rn[3] > 1000
+    assert sr[0] > 0.75
+    assert rn[0] < 0.5
