@@ -680,6 +680,10 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
                 (void)hipGraphDestroy(g);
                 if (e4 != hipSuccess) return seterr(h, PAR_SGNS_EDEVICE, "graph instantiate: %s", hipGetErrorString(e4));
                 it = h->graphs.emplace(k, ge).first;
+                // upload the graph now, so its first launch (the one a caller may
+                // time, when a chunk is trained once) carries no set-up work
+                HIPCHK(h, hipGraphUpload(ge, h->st));
+                HIPCHK(h, hipStreamSynchronize(h->st));
             }
             if (prepare_only) return 0;
             HIPCHK(h, hipEventRecord(h->ev0, h->st));

@@ -207,7 +207,7 @@ __device__ __forceinline__ int alias_draw(const uint2 *tab, int V, U4 rr) {
 // tile of dC or dN holds one dimension of four rows per lane (D layout of a
 // 16x16 MFMA: lane l, register r -> row 4(l/16)+r, column l%16), written back
 // a step pair at a time (write_pair): each row of C takes one write-back per
-// group of kMfU*16 unique negatives (one per batch up to 64), each row of N
+// group of kMfU*16 unique negatives (one per batch up to 32), each row of N
 // one.
 // The batch holds at most kMfI*16 = 32 positions (the host keeps B <= 32 on
 // this path), so every product reads the one snapshot the reference uses.

@@ -103,7 +103,11 @@ __host__ __device__ inline int pow2_ceil(int r) {
 // over distinct banks), one 16 x 16 bf16 tile for transposing the gradient
 // (rows of kMfTS) and two staging tiles of 16 rows x 32 dimensions of N
 // (rows of kMfSS: the four row groups of a column read fall on distinct banks).
-constexpr int kMfU = 4;
+// Two tiles per group (32 negatives): a batch of short sentences draws at most
+// 4 x negative of them at a sentence's edges, and every tile of the group holds
+// registers whether it is used or not (four tiles: 250 VGPRs, two: 206).  A
+// batch with more unique negatives takes one W_in write-back per group.
+constexpr int kMfU = 2;
 constexpr int kMfI = 2;
 constexpr int kMfPad = 4;
 constexpr int kMfTS = 20;
