@@ -347,9 +347,9 @@ __device__ __forceinline__ void store_value_rsrc(__amdgpu_buffer_rsrc_t rw, uint
 }
 
 template <int NP, int WIN, int WOUT>
-__device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t *bat, int isz, int U,
-                                               const int32_t *uneg, const int32_t *umul, char *lds, int lane,
-                                               float alpha_r, double &loss_r, uint32_t &rs) {
+__device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t *bat, int bat_tok, bool bat_in_reg,
+                                               int isz, int U, const int32_t *uneg, const int32_t *umul, char *lds,
+                                               int lane, float alpha_r, double &loss_r, uint32_t &rs) {
     uint16_t *w_in = static_cast<uint16_t *>(a.w_in);
     uint16_t *w_out = static_cast<uint16_t *>(a.w_out);
     const int ld = a.ld, d = a.d, SC = mf_snap_stride(ld);
@@ -375,10 +375,13 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
         // the row are switched off, so nothing lands in the row's padding or
         // the next row.  One wait for all of them follows (such loads count
         // in vmcnt like any other).
-        // (the batch's row ids are read first, lane i holding row i: a load
-        // inside the loop would wait for every row already in flight)
+        // (the batch's row ids are taken first, lane i holding row i: a load
+        // inside the loop would wait for every row already in flight.  For a
+        // sentence of at most 128 tokens they come from the registers that
+        // hold its tokens, bat_tok; a load would wait for the sentence's last
+        // write-back atomics, which retire before it)
         for (int q = lane; 2 * q < ld; q += kWave) *reinterpret_cast<uint32_t *>(snapC + a.B * SC + 2 * q) = 0u;
-        const int bat_id = lane < ni ? bat[lane] : 0;  // ni <= kMfI * 16 <= 64
+        const int bat_id = bat_in_reg ? bat_tok : (lane < ni ? bat[lane] : 0);  // ni <= kMfI * 16 <= 64
         for (int i = 0; i < ni; i++) {
             const uint32_t off = (uint32_t)__builtin_amdgcn_readlane(bat_id, i) * row_bytes + 4u * (uint32_t)lane;
 #pragma unroll
@@ -389,9 +392,6 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
             }
         }
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the snapshot has landed
-        int32_t rowC[kMfI];
-#pragma unroll
-        for (int it = 0; it < kMfI; it++) rowC[it] = it * 16 + r16 < ni ? bat[it * 16 + r16] : -1;
 
         for (int u0 = 0; u0 < U; u0 += 16 * kMfU) {
             const int nu = U - u0 < 16 * kMfU ? U - u0 : 16 * kMfU;
@@ -496,7 +496,11 @@ __device__ __forceinline__ void repulsion_mfma(const FastArgs &a, const int32_t 
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int ip = it * 16 + 4 * g4 + r;
-                    offC[it][r] = ip < ni ? (uint32_t)bat[ip] * row_bytes : kNoRow;
+                    // (every lane shuffles, then selects: a shuffle inside the
+                    // select would run in the lanes with ip < ni only and read
+                    // lanes outside that set)
+                    const uint32_t id = (uint32_t)__shfl(bat_id, ip & (kWave - 1), kWave);
+                    offC[it][r] = ip < ni ? id * row_bytes : kNoRow;
                 }
             }
             // (slices past the last pair read the next row's start or zeros, unused)
@@ -651,6 +655,12 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     char *ring_last = last_in ? ring_lane + kWave * sizeof(float2) * (NP - 1) : reinterpret_cast<char *>(zero_pair);
 
     cnt.tokens = (uint32_t)L;
+    // the sentence's first 128 tokens, lane k holding tokens k and 64 + k (-1
+    // past the end), kept for the repulsion: a sentence of at most 128 tokens
+    // is then checked against its negatives and gives its batch rows without
+    // reloading them (a per-lane load retires after every atomic before it)
+    const bool short_sen = L <= 2 * kWave;
+    int tok_lo = -1, tok_hi = -1;
 
     // Slot of word w among slots still referenced at or after position `from`
     // (-1 if none), first match wins.
@@ -737,6 +747,8 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
         int tb = 0;
         auto tok_block = [&](int base) -> int { return base + lane < L ? sen[base + lane] : -1; };
         int tw0 = tok_block(0), tw1 = tok_block(kWave);
+        tok_lo = tw0;
+        tok_hi = tw1;
         auto tok = [&](int x) -> int {
             const int o = x - tb;
             return o < kWave ? __builtin_amdgcn_readlane(tw0, o) : __builtin_amdgcn_readlane(tw1, o - kWave);
@@ -1001,7 +1013,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             for (int attempt = 1; attempt <= kMaxRedraw; attempt++) {
                 bool hit = false;
                 for (int t0 = 0; t0 < L; t0 += kWave) {
-                    const int tk = t0 + lane < L ? sen[t0 + lane] : -1;
+                    const int tk = short_sen ? (t0 == 0 ? tok_lo : tok_hi) : (t0 + lane < L ? sen[t0 + lane] : -1);
                     const int n = L - t0 < kWave ? L - t0 : kWave;
                     for (int j = 0; j < n; j++) hit |= (cand == __builtin_amdgcn_readlane(tk, j));
                 }
@@ -1035,7 +1047,14 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             U += __builtin_popcountll(fm);
         }
         if constexpr (MF) {  // matrix-core path (bf16 tables, its own kernel instance)
-            repulsion_mfma<NP, WIN, WOUT>(a, sen + lo, isz, U, uneg, umul, lds, lane, alpha_r, loss_r, rs);
+            int bat_tok = 0;  // lane i: token lo + i (sentences of at most 128 tokens)
+            if (short_sen) {
+                const int x = lo + lane;
+                const int t_lo = __shfl(tok_lo, x & (kWave - 1), kWave), t_hi = __shfl(tok_hi, x & (kWave - 1), kWave);
+                bat_tok = x < kWave ? t_lo : t_hi;
+            }
+            repulsion_mfma<NP, WIN, WOUT>(a, sen + lo, bat_tok, short_sen, isz, U, uneg, umul, lds, lane, alpha_r,
+                                          loss_r, rs);
             continue;
         }
         float2 *acc = reinterpret_cast<float2 *>(lds);  // [isz][nq] fp32 W_in deltas
