@@ -519,6 +519,9 @@ constexpr int64_t kMinSentencesPerWave = 16;
 // kernel has no per-block LDS of its own); at most kMaxWavesCu waves per CU
 // (two per SIMD at the kernel's register budget).
 constexpr int kLdsCu = 160 * 1024, kMaxWavesCu = 8, kMaxWpb = 8;
+// log2 slots of a long sentence's token hash set: 256 (sentences up to 192
+// tokens) to 2048 (any sentence)
+constexpr int kTabLog2Min = 8, kTabLog2Max = 11;
 static int blocks_per_cu(int lds_per_wave, int wpb) {
     int b = kLdsCu / (lds_per_wave * wpb);
     if (b * wpb > kMaxWavesCu) b = kMaxWavesCu / wpb;
@@ -657,6 +660,23 @@ static int train_impl(par_sgns_handle h, const par_sgns_train_args *a, par_sgns_
         if (f.grab > 64) return seterr(h, PAR_SGNS_EINVAL, "sentences_per_dequeue must be 0..64");
         if ((size_t)f.lds_per_wave * wpb > 160 * 1024)
             return seterr(h, PAR_SGNS_EINVAL, "window ring does not fit in LDS (%d bytes per wave)", f.lds_per_wave);
+        {   // the token hash set of long sentences (rejection check), after the
+            // batch's negatives: in the share's free bytes, or in a larger share
+            // when the persistent grid fits the device at once with it; 2^11
+            // slots hold any sentence (at most 1000 tokens) at a load below 1/2
+            f.tab_off = (f.lds_negs + 12 * h->kmax + 15) & ~15;
+            int64_t cap = f.lds_per_wave;
+            if (pers) {
+                const int cus = h->cu_count > 0 ? h->cu_count : 256;
+                const int64_t per_cu = ((int64_t)blocks + cus - 1) / cus * wpb;  // resident waves per CU
+                const int64_t fit = (kLdsCu / per_cu) & ~15;
+                if (fit > cap) cap = std::min<int64_t>(fit, f.tab_off + (4 << kTabLog2Max));
+            }
+            f.tab_log2 = 0;
+            for (int k = kTabLog2Max; k >= kTabLog2Min && !f.tab_log2; k--)
+                if (f.tab_off + (4 << k) <= cap) f.tab_log2 = k;
+            if (f.tab_log2 && f.tab_off + (4 << f.tab_log2) > f.lds_per_wave) f.lds_per_wave = f.tab_off + (4 << f.tab_log2);
+        }
         shape_wpb = wpb;
         shape_blocks = pers ? blocks : (int)((a->sent_end - a->sent_begin + wpb - 1) / wpb);
         shape_grab = pers ? f.grab : 0;

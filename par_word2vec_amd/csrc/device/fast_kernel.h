@@ -43,6 +43,12 @@ namespace sgns {
 namespace fast {
 
 constexpr int kMaxRedraw = 64;
+// Open-addressing hash set of a sentence's token ids in LDS, for the
+// in-sentence rejection check of sentences longer than the 128 tokens kept in
+// registers: slots hold a token id or kTabEmpty; a sentence uses the set when
+// it holds at most 3/4 of the slots.
+constexpr int32_t kTabEmpty = -1;
+__device__ __forceinline__ uint32_t tab_hash(int32_t w, int log2) { return ((uint32_t)w * 0x9E3779B1u) >> (32 - log2); }
 constexpr int kUnusedSlot = -(1 << 30);
 
 // Rounding of bf16 write-backs per table (par_sgns_rounding): stochastic for
@@ -977,6 +983,34 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     const float alpha_r = L == 1 ? alpha0 : alpha;
     int32_t *uneg = negs + a.kmax;  // unique negatives of the batch
     int32_t *umul = uneg + a.kmax;  // their multiplicities
+    // A sentence longer than its token registers is checked against its
+    // negatives through a hash set of its tokens, built here once: a probe or
+    // two per candidate instead of a pass over every token of the sentence
+    // per check.  The set holds exactly the sentence's words, so the check
+    // rejects what the pass does.
+    // (one value stays live over the batches: log2 of the slots, 0 when the
+    // sentence takes the token pass instead)
+    const int tab_lg = !short_sen && !planned && a.tab_log2 > 0 && 4 * L <= 3 << a.tab_log2 ? a.tab_log2 : 0;
+    if (tab_lg) {
+        int32_t *tab = reinterpret_cast<int32_t *>(lds + a.tab_off);
+        const uint32_t tab_mask = (1u << tab_lg) - 1u;
+        int4 *t4 = reinterpret_cast<int4 *>(tab);
+        for (int i = lane; i < (1 << tab_lg) / 4; i += kWave) t4[i] = make_int4(kTabEmpty, kTabEmpty, kTabEmpty, kTabEmpty);
+        __builtin_amdgcn_wave_barrier();
+        for (int t0 = 0; t0 < L; t0 += kWave) {
+            const int w = t0 + lane < L ? sen[t0 + lane] : kTabEmpty;
+            uint32_t h = tab_hash(w, tab_lg);
+            bool pend = w != kTabEmpty;
+            while (__builtin_amdgcn_ballot_w64(pend)) {  // linear probing; a word already there is done
+                if (pend) {
+                    const int old = atomicCAS(tab + h, kTabEmpty, w);
+                    if (old == kTabEmpty || old == w) pend = false;
+                    else h = (h + 1) & tab_mask;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
     for (int bi = 0; bi < nbat; bi++) {
         const int lo = bi * B;
         const int isz = L - lo < B ? L - lo : B;
@@ -1012,10 +1046,33 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             // vocabulary, and then the last candidate is kept.
             for (int attempt = 1; attempt <= kMaxRedraw; attempt++) {
                 bool hit = false;
-                for (int t0 = 0; t0 < L; t0 += kWave) {
+                if (tab_lg) {
+                    const int32_t *tab = reinterpret_cast<const int32_t *>(lds + a.tab_off);
+                    const uint32_t tab_mask = (1u << tab_lg) - 1u;
+                    uint32_t h = tab_hash(cand, tab_lg);
+                    bool pend = live;
+                    while (__builtin_amdgcn_ballot_w64(pend)) {
+                        if (pend) {
+                            const int v = tab[h];
+                            if (v == cand) hit = true;
+                            if (v == cand || v == kTabEmpty) pend = false;
+                            else h = (h + 1) & tab_mask;
+                        }
+                    }
+                } else for (int t0 = 0; t0 < L; t0 += kWave) {
                     const int tk = short_sen ? (t0 == 0 ? tok_lo : tok_hi) : (t0 + lane < L ? sen[t0 + lane] : -1);
                     const int n = L - t0 < kWave ? L - t0 : kWave;
-                    for (int j = 0; j < n; j++) hit |= (cand == __builtin_amdgcn_readlane(tk, j));
+                    // runs of 16 constant lanes with one uniform test per run: a
+                    // loop per lane costs a branch and two scalar-vector round
+                    // trips per token; lanes past the sentence hold -1, which
+                    // matches no candidate
+#pragma unroll
+                    for (int j0 = 0; j0 < kWave; j0 += 16) {
+                        if (j0 < n) {
+#pragma unroll
+                            for (int j = 0; j < 16; j++) hit |= (cand == __builtin_amdgcn_readlane(tk, j0 + j));
+                        }
+                    }
                 }
                 const bool pending = live && hit;
                 if (__builtin_amdgcn_ballot_w64(pending) == 0) break;

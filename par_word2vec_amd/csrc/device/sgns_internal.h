@@ -41,6 +41,7 @@ struct FastArgs {
     int32_t ld, d, nq, W, R, NEG, B, V, kmax;
     int32_t lds_per_wave;         // bytes
     int32_t lds_negs;             // byte offset of the batch's negatives in a wave's LDS (after the snapshot)
+    int32_t tab_off, tab_log2;    // the token hash set of a long sentence: byte offset, log2 slots (0: none)
     int32_t persistent;
     int32_t grab;                 // persistent: queue slots (sentences) a wave takes per dequeue, >= 1
     int32_t rounding;             // bf16 write-back: 0 stochastic, 1 nearest

@@ -54,6 +54,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import philox_schedule as PS
 from par_word2vec_amd import corpora, evaluate as E, host as H
 from par_word2vec_amd.device import DeviceError, SgnsDevice
 from par_word2vec_amd.trainer import chunk_bounds
@@ -363,6 +364,59 @@ def test_hogwild_philox_loss_curve_and_recall(zipf1m, oracle_curves, precision, 
     assert np.all(np.abs(ea) < ba) and np.all(np.abs(er) < br)
     if precision == "fp32":
         assert abs(recall / ref["recall"] - 1) < 0.01
+
+
+@pytest.mark.parametrize("mode,precision,rep,negative,epoch", [
+    ("persistent", "fp32", "vector", 5, 0),   # long sentences checked through the LDS hash set
+    ("persistent", "bf16", "matrix", 10, 1),  # two blocks of 64 candidates; the epoch in the keys
+    ("plain", "fp32", "vector", 10, 0),       # a plain wave's share has no room for the set: the token scan
+    ("plain", "bf16", "matrix", 5, 1),
+])
+def test_device_draws_match_host_restatement(mode, precision, rep, negative, epoch):
+    """The Hogwild kernel drawing its own schedule (window shrinks, batch
+    windows, alias-table negatives rejected while in the sentence, float32
+    learning rates) trains, one sentence at a time, exactly what it trains
+    when it replays the schedule the host restatement predicts from the same
+    key: equal tables bit for bit.  Sentences run longest first, so the
+    persistent queue keeps their ids; plain mode runs them as one launch each
+    (min_sentences_per_wave = the sentence count)."""
+    print(f"[draws] mode={mode} {precision} {rep} neg={negative} epoch={epoch}", flush=True)
+    dim, window, batch, a0, seed = 64, 5, 24, 0.025, 0x1234567890ABCDEF
+    tokens, off, raw, T, counts = PS.rejection_corpus()
+    V, n = counts.size, off.size - 1
+    entries = H.unigram_entries(counts)
+    plan, pst = PS.predict_plan(tokens, off, raw, entries, window=window, negative=negative, batch_size=batch,
+                                seed=seed, epoch=epoch, iters=epoch + 1, alpha=a0, train_words=T)
+    w_in0, _ = H.init_net(V, dim)
+    w_out0 = (np.random.default_rng(1).random((V, dim)).astype(np.float32) - 0.5) / dim
+    if precision == "bf16":  # representable, so both runs start from the same tables
+        w_in0, w_out0 = ((x.view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32) for x in (w_in0, w_out0))
+    out = {}
+    for src in ("drawn", "planned"):
+        with SgnsDevice(V, dim, window, negative, batch, precision) as dev:
+            dev.set_exp_table(H.exp_table())
+            dev.set_weights(w_in0, w_out0)
+            if src == "drawn":
+                dev.set_sampler(entries)
+                dev.set_corpus(tokens, off, raw)
+                run = dict(mode=mode, min_sentences_per_wave=n if mode == "plain" else 0)
+            else:
+                dev.set_plan(plan)
+                run = dict(mode="persistent", planned=True)
+            if run["mode"] == "persistent":
+                run.update(blocks=1, waves_per_block=1, sentences_per_dequeue=1)
+            m = run.pop("mode")
+            st = dev.train(m, 0, n, a0, T, iters=epoch + 1, epoch=epoch, seed=seed, compute_loss=True,
+                           repulsion=rep, **run)
+            out[src] = dev.get_weights() + (st,)
+    (di, do, ds), (pi, po, ps) = out["drawn"], out["planned"]
+    for k in ("tokens", "pairs", "batches", "negatives", "rep_pairs"):
+        assert ds[k] == ps[k], k
+    assert ds["batches"] == pst["batches"] and ds["negatives"] == pst["negatives"]
+    np.testing.assert_array_equal(di, pi)
+    np.testing.assert_array_equal(do, po)
+    assert ds["loss_attr"] == ps["loss_attr"] and ds["loss_rep"] == ps["loss_rep"]
+    assert np.abs(di - w_in0).max() > 0 and np.abs(do - w_out0).max() > 0
 
 
 def test_plain_mode_counts(zipf1m):
