@@ -339,7 +339,7 @@ def main() -> int:
                     help="row write-back of both variants: W_in atomics and W_out plain stores (default), all "
                          "atomics, or plain stores of W_in / both tables (bf16 matrix-core path only)")
     ap.add_argument("--min-sentences-per-wave", type=int, default=0,
-                    help="Hogwild staleness bound of both variants: persistent grid floor (0 = 16 sentences per "
+                    help="Hogwild staleness bound of both variants: persistent grid floor (0 = 24 sentences per "
                          "wave), plain mode at most n / this sentences in flight (0 = one launch)")
     ap.add_argument("--rounding", default="stochastic", choices=["stochastic", "nearest", "nearest_out", "nearest_in"],
                     help="bf16 write-back rounding: stochastic (default) or round-to-nearest for both tables, or "

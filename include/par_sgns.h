@@ -112,7 +112,7 @@ typedef struct par_sgns_train_args {
                                       dequeue, 1..64; 0 = tuned default (1 untuned) */
     int32_t min_sentences_per_wave; /* Hogwild staleness bound.  Persistent mode, default
                                        grid: at least this many sentences per wavefront
-                                       (fewer waves on small ranges); 0 = 16.  Plain mode,
+                                       (fewer waves on small ranges); 0 = 24.  Plain mode,
                                        if > 0: at most range / this many sentences in flight
                                        (consecutive launches of that many sentences where
                                        fewer than that fit on the device at once); 0 = one

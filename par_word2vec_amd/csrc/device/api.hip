@@ -512,8 +512,13 @@ static int ensure_order(par_sgns_handle h, int64_t b, int64_t e) {
 // Default persistent grid: as many resident waves as the LDS allows, but at
 // least kMinSentencesPerWave sentences per wave.  Hogwild quality depends on
 // each row seeing a sequence of updates; a small corpus spread over every
-// resident wave of the chip turns a pass into a few huge stale steps.
-constexpr int64_t kMinSentencesPerWave = 16;
+// resident wave of the chip turns a pass into a few huge stale steps.  How
+// stale a step is depends on the chip's update rate as well as on the number
+// of waves: on an epoch of 17,000 sentences of 1,000 words (bf16, 570 kept
+// tokens per sentence) 16 sentences per wave kept planted-cluster recall@10
+// at 0.99 while the kernel ran at 300 Mwords/s, and fell to 0.96 once it ran
+// at 500; 24 keeps 0.998 at 355 Mwords/s.
+constexpr int64_t kMinSentencesPerWave = 24;
 
 // Resident blocks per CU for a block of wpb waves: 160 KB of LDS per CU (the
 // kernel has no per-block LDS of its own); at most kMaxWavesCu waves per CU

@@ -48,7 +48,7 @@ void usage() {
         "  -mode det|plain|persistent (persistent) -precision fp32|bf16 (fp32) -seed <int> -device <int>\n"
         "  -chunk-words <int> (10000000) -time-file <file> (PAR_Word2Vec_gpu_time)\n"
         "  -autotune <runs> (0): persistent mode, tune the launch shape on the first chunk first\n"
-        "  -min-sentences-per-wave <int>: staleness bound; persistent mode: sentences per wave (16);\n"
+        "  -min-sentences-per-wave <int>: staleness bound; persistent mode: sentences per wave (24);\n"
         "    plain mode: at most n / this sentences in flight (0: off)\n"
         "  -write-back atomic|store_out|store_in|store|hot (atomic) -hot-rows <int>: bf16 row write-back\n"
         "  -rounding stochastic|nearest|nearest_out|nearest_in (stochastic): bf16 write-back rounding\n"

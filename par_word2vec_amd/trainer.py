@@ -45,7 +45,7 @@ class Options:
     use_graph: bool = True
     compute_loss: bool = False
     autotune: int = 0             # persistent mode: tune the launch shape on the first chunk (runs per shape)
-    min_sentences_per_wave: int = 0  # persistent mode: staleness floor of the default grid (0 = 16)
+    min_sentences_per_wave: int = 0  # persistent mode: staleness floor of the default grid (0 = 24)
 
 
 def chunk_bounds(sent_raw: np.ndarray, chunk_words: int) -> list[tuple[int, int]]:

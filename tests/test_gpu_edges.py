@@ -141,15 +141,15 @@ def test_sentences_per_wave_floor(ragged, floor):
     with _dev(vocab, "bf16", *H.init_net(len(vocab), DIM)) as dev:
         dev.set_sampler(H.unigram_entries(counts))
         dev.set_corpus(csr.tokens, csr.sent_off, csr.sent_raw)
-        base = dev.train("persistent", 0, n, 0.025, T, seed=3, use_graph=True)  # default floor, 16
+        base = dev.train("persistent", 0, n, 0.025, T, seed=3, use_graph=True)  # default floor, 24
         st = dev.train("persistent", 0, n, 0.025, T, seed=3, use_graph=True, min_sentences_per_wave=floor)
         assert st["tokens"] == csr.tokens.size
         waves, base_waves = st["blocks"] * st["waves_per_block"], base["blocks"] * base["waves_per_block"]
-        assert base_waves <= max(1, n // 16) * base["waves_per_block"]
+        assert base_waves <= max(1, n // 24) * base["waves_per_block"]
         assert waves <= max(1, n // floor)
         if floor == 100000:
             assert waves == 1
-        if floor < 16:
+        if floor < 24:
             assert waves > base_waves
         with pytest.raises(DeviceError):
             dev.train("persistent", 0, n, 0.025, T, min_sentences_per_wave=-1)
