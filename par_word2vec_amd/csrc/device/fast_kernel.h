@@ -768,31 +768,46 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             if (sl < 0) sl = find_free(0);
             bind(c, sl, w);
         }
-        {  // then fetch every bound slot, four rows in flight
+        // vq0/vq1: W_out rows of centres p and p+1; pq: the W_in row entering at
+        // p+1+W, loaded only if its word then holds no slot (np_slot < 0).  With
+        // one pair per lane they go out before the first window's rows, which
+        // follow eight at a time, so a window of up to eight positions is set up
+        // in one round trip; wider rows keep four rows in flight and load the
+        // centre rows after them (at two waves per SIMD the other wave hides
+        // those round trips, and eight wide rows cost registers).
+        PT vq0[NP], vq1[NP], pq[NP];
+        int np_slot = -1;
+        if constexpr (NP == 1) {
+            load_row<T, NP>(w_out, tok(0), ld, nq, lane, vq0);
+            if (1 < L && tok(1) != tok(0)) load_row<T, NP>(w_out, tok(1), ld, nq, lane, vq1);
+            np_slot = 1 + W < L ? find_slot(tok(1 + W), -W) : -1;
+            if (1 + W < L && np_slot < 0) load_row<T, NP>(w_in, tok(1 + W), ld, nq, lane, pq);
+        }
+        {  // fetch every bound slot
+            constexpr int kFetch = NP == 1 ? 8 : 4;
             uint64_t mask = __builtin_amdgcn_ballot_w64(lane < R && slot_last >= 0);
             while (mask) {
-                int sl4[4];
+                int slk[kFetch];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    sl4[k] = mask ? (int)__builtin_ctzll(mask) : -1;
+                for (int k = 0; k < kFetch; k++) {
+                    slk[k] = mask ? (int)__builtin_ctzll(mask) : -1;
                     mask &= mask ? mask - 1 : 0;
                 }
-                PT rows[4][NP];
+                PT rows[kFetch][NP];
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (sl4[k] >= 0) load_row<T, NP>(w_in, word_of(sl4[k]), ld, nq, lane, rows[k]);
+                for (int k = 0; k < kFetch; k++)
+                    if (slk[k] >= 0) load_row<T, NP>(w_in, word_of(slk[k]), ld, nq, lane, rows[k]);
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (sl4[k] >= 0) fill(sl4[k], rows[k]);
+                for (int k = 0; k < kFetch; k++)
+                    if (slk[k] >= 0) fill(slk[k], rows[k]);
             }
         }
-        // vq0/vq1: W_out rows of centres p and p+1; pq: the W_in row entering at
-        // p+1+W, loaded only if its word then holds no slot (np_slot < 0)
-        PT vq0[NP], vq1[NP], pq[NP];
-        load_row<T, NP>(w_out, tok(0), ld, nq, lane, vq0);
-        if (1 < L && tok(1) != tok(0)) load_row<T, NP>(w_out, tok(1), ld, nq, lane, vq1);
-        int np_slot = 1 + W < L ? find_slot(tok(1 + W), -W) : -1;
-        if (1 + W < L && np_slot < 0) load_row<T, NP>(w_in, tok(1 + W), ld, nq, lane, pq);
+        if constexpr (NP != 1) {
+            load_row<T, NP>(w_out, tok(0), ld, nq, lane, vq0);
+            if (1 < L && tok(1) != tok(0)) load_row<T, NP>(w_out, tok(1), ld, nq, lane, vq1);
+            np_slot = 1 + W < L ? find_slot(tok(1 + W), -W) : -1;
+            if (1 + W < L && np_slot < 0) load_row<T, NP>(w_in, tok(1 + W), ld, nq, lane, pq);
+        }
         // centre 0's row, settled here: every iteration then enters the loop
         // with the same loads in flight (the next centres' rows only)
 #pragma unroll
