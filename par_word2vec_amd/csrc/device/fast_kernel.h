@@ -647,8 +647,12 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     int slot_word = -1, slot_last = kUnusedSlot, pos_slot = 0;
     // byte stride of a ring slot, in a vector register: the pair loop's
     // address is then one multiply-add with no scalar reload of nq
-    uint32_t slot_bytes = (uint32_t)nq * (uint32_t)sizeof(float2);
-    asm volatile("" : "+v"(slot_bytes));
+    // (nq is moved into a vector register and the stride derived from it, so
+    // the stride's low three bits stay known: the ring addresses are then
+    // 8-byte aligned 32-bit LDS addresses, one 24-bit multiply-add each)
+    uint32_t nq_v = (uint32_t)nq;
+    asm volatile("" : "+v"(nq_v));
+    const uint32_t slot_bytes = nq_v * (uint32_t)sizeof(float2);
     char *ring_lane = reinterpret_cast<char *>(ring_u + lane);
     // A lane whose last pair lies past the row (q >= nq) points that pair at a
     // zero pair of its own in the slack after the ring, for every slot: its
@@ -656,9 +660,13 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     // pair loop needs no lane predicate or mask.
     const bool last_in = lane + kWave * (NP - 1) < nq;
     float2 *zero_pair = reinterpret_cast<float2 *>(lds + (size_t)R * nq * (sizeof(float2) + sizeof(PT))) + lane;
-    uint32_t slot_bytes_last = last_in ? slot_bytes : 0u;
-    asm volatile("" : "+v"(slot_bytes_last));
+    const uint32_t slot_bytes_last = (last_in ? nq_v : 0u) * (uint32_t)sizeof(float2);
     char *ring_last = last_in ? ring_lane + kWave * sizeof(float2) * (NP - 1) : reinterpret_cast<char *>(zero_pair);
+    typedef __attribute__((address_space(3))) char lchar;
+    typedef float __attribute__((ext_vector_type(2))) vf2;
+    typedef __attribute__((address_space(3))) vf2 lfloat2;
+    lchar *ring_lane_l = (lchar *)ring_lane;
+    lchar *ring_last_l = (lchar *)ring_last;
 
     cnt.tokens = (uint32_t)L;
     // the sentence's first 128 tokens, lane k holding tokens k and 64 + k (-1
@@ -854,12 +862,15 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 auto pos = [&](int k) { return lo + k + (lo + k >= p ? 1 : 0); };
                 // ring entry t of this lane in a slot (the last one: see zero_pair)
                 auto ring_at = [&](int slot, int t) {
-                    return t == NP - 1 ? reinterpret_cast<float2 *>(ring_last + (uint32_t)slot * slot_bytes_last)
-                                       : reinterpret_cast<float2 *>(ring_lane + (uint32_t)slot * slot_bytes) + kWave * t;
+                    return t == NP - 1 ? (lfloat2 *)(ring_last_l + __umul24((uint32_t)slot, slot_bytes_last))
+                                       : (lfloat2 *)(ring_lane_l + __umul24((uint32_t)slot, slot_bytes)) + kWave * t;
                 };
                 auto read_row = [&](int slot, float2 (&u)[NP]) {
 #pragma unroll
-                    for (int t = 0; t < NP; t++) u[t] = *ring_at(slot, t);
+                    for (int t = 0; t < NP; t++) {
+                        const vf2 x = *ring_at(slot, t);
+                        u[t] = make_float2(x.x, x.y);
+                    }
                 };
                 auto dot = [&](const float2 (&x)[NP], const float2 (&y)[NP]) {  // one FMA chain
                     float s = x[0].x * y[0].x;
@@ -891,7 +902,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
 #pragma unroll
                     for (int t = 0; t < NP; t++) {
                         nw[t] = make_float2(cur[t].x + g * v[t].x, cur[t].y + g * v[t].y);
-                        *ring_at(slot, t) = nw[t];
+                        *ring_at(slot, t) = vf2{nw[t].x, nw[t].y};
                     }
 #pragma unroll
                     for (int t = 0; t < NP; t++) {  // v_{k+1}
