@@ -419,6 +419,44 @@ def test_device_draws_match_host_restatement(mode, precision, rep, negative, epo
     assert np.abs(di - w_in0).max() > 0 and np.abs(do - w_out0).max() > 0
 
 
+@pytest.mark.parametrize("precision,rep", [("bf16", "matrix"), ("fp32", "vector")])
+def test_device_draws_several_waves_counts(precision, rep):
+    """Several wavefronts at once, each checking long sentences against its own
+    hash set in its share of the block's LDS (one block of four waves; the
+    default grid): the schedule they draw is the host restatement's, whatever
+    the interleaving -- equal batch, negative, negative-position and
+    attraction-pair counts -- and the tables stay finite."""
+    dim, window, negative, batch, a0, seed = 64, 5, 10, 24, 0.025, 0x5EED
+    tokens, off, raw, T, counts = PS.rejection_corpus()
+    V, n = counts.size, off.size - 1
+    entries = H.unigram_entries(counts)
+    plan, pst = PS.predict_plan(tokens, off, raw, entries, window=window, negative=negative, batch_size=batch,
+                                seed=seed, epoch=0, iters=1, alpha=a0, train_words=T)
+    rep_pairs = pairs = 0
+    for s in range(n):
+        L = int(off[s + 1] - off[s])
+        for bi in range(int(plan.batch_off[s + 1] - plan.batch_off[s])):
+            bo = plan.batch_off[s] + bi
+            rep_pairs += int(plan.neg_off[bo + 1] - plan.neg_off[bo]) * min(batch, L - bi * batch)
+        for p in range(L):
+            b = int(plan.b[off[s] + p])
+            pairs += min(p + window - b, L - 1) - max(p - window + b, 0)
+    w_in0, w_out0 = H.init_net(V, dim)
+    for blocks, wpb in ((1, 4), (0, 0)):
+        print(f"[draws] {precision} {rep} blocks={blocks} waves_per_block={wpb}", flush=True)
+        with SgnsDevice(V, dim, window, negative, batch, precision) as dev:
+            dev.set_exp_table(H.exp_table())
+            dev.set_weights(w_in0, w_out0)
+            dev.set_sampler(entries)
+            dev.set_corpus(tokens, off, raw)
+            st = dev.train("persistent", 0, n, a0, T, seed=seed, blocks=blocks, waves_per_block=wpb,
+                           sentences_per_dequeue=1, repulsion=rep)
+            wi, wo = dev.get_weights()
+        assert st["tokens"] == tokens.size and st["batches"] == pst["batches"]
+        assert st["negatives"] == pst["negatives"] and st["rep_pairs"] == rep_pairs and st["pairs"] == pairs
+        assert np.isfinite(wi).all() and np.isfinite(wo).all() and np.abs(wo - w_out0).max() > 0
+
+
 def test_plain_mode_counts(zipf1m):
     """One workgroup per sentence: every sentence and token is trained once;
     counters are consistent with the CSR."""
