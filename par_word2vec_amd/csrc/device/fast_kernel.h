@@ -1140,6 +1140,19 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                                           loss_r, rs);
             continue;
         }
+        // the batch's row ids, lane i holding row i (isz <= B <= 64): from the
+        // registers that hold a short sentence's tokens, else one load per
+        // batch -- a per-row load would put a dependent round trip, behind the
+        // wave's last atomics, before every row
+        int bat_id;
+        if (short_sen) {
+            const int x = lo + lane;
+            const int t_lo = __shfl(tok_lo, x & (kWave - 1), kWave), t_hi = __shfl(tok_hi, x & (kWave - 1), kWave);
+            bat_id = x < kWave ? t_lo : t_hi;
+        } else {
+            bat_id = lane < isz ? sen[lo + lane] : 0;
+        }
+        auto row_id = [&](int i) { return __builtin_amdgcn_readlane(bat_id, i); };
         float2 *acc = reinterpret_cast<float2 *>(lds);  // [isz][nq] fp32 W_in deltas
         for (int i = 0; i < isz; i++) {
 #pragma unroll
@@ -1177,28 +1190,31 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 for (int t = 0; t < NP; t++) dn[k][t] = make_float2(0.0f, 0.0f);
             }
             PT ra[NP], rb[NP];
-            load_row<T, NP>(w_in, sen[lo], ld, nq, lane, ra);
+            load_row<T, NP>(w_in, row_id(0), ld, nq, lane, ra);
             if (kb + NB < U) load_block(kb + NB);
             const float mlane = lane < nbk ? (float)umul[kb + lane] : 0.0f;  // loss: lane k <-> negative k
             for (int i = 0; i < isz; i++) {
                 float2 c[NP];
                 if ((i & 1) == 0) {
-                    if (i + 1 < isz) load_row<T, NP>(w_in, sen[lo + i + 1], ld, nq, lane, rb);
+                    if (i + 1 < isz) load_row<T, NP>(w_in, row_id(i + 1), ld, nq, lane, rb);
                     to_f<T, NP>(ra, c);
                 } else {
-                    if (i + 1 < isz) load_row<T, NP>(w_in, sen[lo + i + 1], ld, nq, lane, ra);
+                    if (i + 1 < isz) load_row<T, NP>(w_in, row_id(i + 1), ld, nq, lane, ra);
                     to_f<T, NP>(rb, c);
                 }
-                float g[NB];
+                float g[NB], part[NB], fk[NB];
                 float fbuf = 0.0f;  // lane k: logit against negative k (loss only)
 #pragma unroll
                 for (int k = 0; k < NB; k++) {
-                    float part = 0.0f;
+                    part[k] = 0.0f;
 #pragma unroll
-                    for (int t = 0; t < NP; t++) part += nv[k][t].x * c[t].x + nv[k][t].y * c[t].y;
-                    const float f = wave_sum(part);
-                    g[k] = k < nbk ? grad_label0_r(f, alpha_r) : 0.0f;
-                    fbuf = lane == k ? f : fbuf;
+                    for (int t = 0; t < NP; t++) part[k] += nv[k][t].x * c[t].x + nv[k][t].y * c[t].y;
+                }
+                wave_sum_n<NB>(part, fk);  // the block's logits in one reduction tree
+#pragma unroll
+                for (int k = 0; k < NB; k++) {
+                    g[k] = k < nbk ? grad_label0_r(fk[k], alpha_r) : 0.0f;
+                    fbuf = lane == k ? fk[k] : fbuf;
                 }
                 if (a.compute_loss && lane < nbk) loss_r += (double)mlane * (double)softplusf(fbuf);
                 float2 dc[NP];
@@ -1241,7 +1257,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
             float2 c[NP], dc[NP];
             if constexpr (sizeof(T) == 2) {
                 PT r[NP];
-                load_row<T, NP>(w_in, sen[lo + i], ld, nq, lane, r);
+                load_row<T, NP>(w_in, row_id(i), ld, nq, lane, r);
                 to_f<T, NP>(r, c);
             } else {
 #pragma unroll
@@ -1252,7 +1268,7 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 const int q = lane + kWave * t;
                 dc[t] = t < NP - 1 || q < nq ? acc[i * nq + q] : make_float2(0.0f, 0.0f);
             }
-            store_delta<T, NP>(w_in, sen[lo + i], ld, nq, lane, c, dc, rs, nearest_in(a));
+            store_delta<T, NP>(w_in, row_id(i), ld, nq, lane, c, dc, rs, nearest_in(a));
         }
     }
 }

@@ -73,6 +73,52 @@ __device__ __forceinline__ void wave_sum2(float a, float b, float &sa, float &sb
     sb = readlane_f(v, 63);
 }
 
+// N sums over the 64 lanes in one tree (N <= 8).  permlane32_swap pairs the
+// values (each register then holds two values' half-sums, one per 32-lane
+// half), permlane16_swap pairs those registers (each 16-lane row then holds
+// one value's quarter-sums), and the row butterfly finishes every row; the
+// rows of register j hold values 4j, 4j+2, 4j+1, 4j+3.  A value without a
+// partner is paired with zeros.
+__device__ __forceinline__ float2 swap32(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                                    false, false);
+    const uint32_t x = r[0], y = r[1];
+    return make_float2(__builtin_bit_cast(float, x), __builtin_bit_cast(float, y));
+}
+__device__ __forceinline__ float2 swap16(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b),
+                                                    false, false);
+    const uint32_t x = r[0], y = r[1];
+    return make_float2(__builtin_bit_cast(float, x), __builtin_bit_cast(float, y));
+}
+template <int N>
+__device__ __forceinline__ void wave_sum_n(const float (&v)[N], float (&out)[N]) {
+    static_assert(N >= 1 && N <= 8, "wave_sum_n: 1..8 values");
+    constexpr int M1 = (N + 1) / 2, M2 = (M1 + 1) / 2;
+    float h[M1];  // half-sums: value 2j in lanes 0-31, 2j+1 in lanes 32-63
+#pragma unroll
+    for (int j = 0; j < M1; j++) {
+        const float2 p = swap32(v[2 * j], 2 * j + 1 < N ? v[2 * j + 1] : 0.0f);
+        h[j] = p.x + p.y;
+    }
+    float q[M2];  // quarter-sums: rows 0..3 hold values 4j, 4j+2, 4j+1, 4j+3
+#pragma unroll
+    for (int j = 0; j < M2; j++) {
+        const float2 p = swap16(h[2 * j], 2 * j + 1 < M1 ? h[2 * j + 1] : 0.0f);
+        float t = p.x + p.y;
+        t += dpp<0xB1>(t);
+        t += dpp<0x4E>(t);
+        t += dpp<0x141>(t);
+        t += dpp<0x140>(t);
+        q[j] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const int j = k / 4, r = k % 4;
+        out[k] = readlane_f(q[j], r == 0 ? 0 : r == 1 ? 32 : r == 2 ? 16 : 48);
+    }
+}
+
 __device__ __forceinline__ int64_t readfirst_i64(int64_t v) {
     int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)(uint64_t)v);
     int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));

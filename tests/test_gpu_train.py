@@ -165,8 +165,14 @@ def test_hogwild_kernel_exact_schedule(precision_dim, window, negative, batch, s
         assert fs[k] == ds[k]
     # the float-reordering error of a repulsion sum grows like the square root
     # of its length, K <= (2W-1)*negative terms per batch row: 1e-6 up to 90
-    # terms (every workload of the reference's flags), scaled beyond
-    rtol = 1e-6 * max(1.0, ((2 * window - 1) * negative / 90) ** 0.5)
+    # terms (every workload of the reference's flags), scaled beyond.  Past
+    # 300 terms the epoch's loss sum turns chaotic on this small vocabulary:
+    # at 704 negatives per batch, one element of W_in moved by one ulp moves
+    # the attraction loss from 1.8e-8 to 4.3e-7 relative (the tables stay
+    # within their bar above), and a different summation tree for the
+    # repulsion's logits gives 3.7e-6; the bar there is 1e-5
+    K = (2 * window - 1) * negative
+    rtol = 1e-6 * max(1.0, (K / 90) ** 0.5) if K <= 300 else 1e-5
     np.testing.assert_allclose(fs["loss_attr"], ds["loss_attr"], rtol=rtol)
     np.testing.assert_allclose(fs["loss_rep"], ds["loss_rep"], rtol=rtol)
 
