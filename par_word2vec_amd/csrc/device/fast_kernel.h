@@ -892,12 +892,19 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
                 auto step = [&](int k, float2 (&cur)[NP], float2 (&nxt)[NP], auto more, auto loss) {
                     int slot_n = slot;
                     float An = 0.0f, Bn = 0.0f;
-                    if constexpr (decltype(more)::value) {  // u_{k+1}.v_k and u_{k+1}.u_k
+                    // the next row's LDS read goes out first and the sigmoid
+                    // of f_k covers its latency; the scheduling barriers
+                    // keep that order (left free, the compiler waits for the
+                    // read and then interleaves the sigmoid with the sums)
+                    if constexpr (decltype(more)::value) {
                         slot_n = slot_of(pos(k + 1));
                         read_row(slot_n, nxt);
-                        wave_sum2(dot(nxt, v), dot(nxt, cur), An, Bn);
                     }
+                    __builtin_amdgcn_sched_barrier(0);
                     const float g = grad_label1_r(f, alpha_p);
+                    __builtin_amdgcn_sched_barrier(0);
+                    if constexpr (decltype(more)::value)  // u_{k+1}.v_k and u_{k+1}.u_k
+                        wave_sum2(dot(nxt, v), dot(nxt, cur), An, Bn);
                     float2 nw[NP];
 #pragma unroll
                     for (int t = 0; t < NP; t++) {
