@@ -53,6 +53,19 @@ constexpr int kUnusedSlot = -(1 << 30);
 
 // Rounding of bf16 write-backs per table (par_sgns_rounding): stochastic for
 // both, round-to-nearest for both, or nearest for one table only.
+// A sentence's wave-uniform metadata (queue order, offsets, rates), which
+// the kernel never writes.  On the matrix-core instances with two or more
+// pairs per lane it is loaded through the scalar cache: the scalar counter
+// tracks those loads, so waiting for them does not wait for this wave's
+// vector atomics to retire (the previous sentence's write-backs).  On the
+// other instances the same loads moved the compiler's register allocation
+// the wrong way (with one pair per lane it reloaded kernel arguments inside
+// the centre loop), so there they stay vector loads.
+template <int NP, bool MF, typename X>
+__device__ __forceinline__ X load_meta(const X *p) {
+    if constexpr (MF && NP >= 2) return *(const __attribute__((address_space(4))) X *)p;
+    else return *p;
+}
 __device__ __forceinline__ int nearest_in(const FastArgs &a) { return a.rounding == 1 || a.rounding == 3; }
 __device__ __forceinline__ int nearest_out(const FastArgs &a) { return a.rounding == 1 || a.rounding == 2; }
 
@@ -625,13 +638,14 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
     const __amdgpu_buffer_rsrc_t rout_w = __builtin_amdgcn_make_buffer_rsrc(a.w_out, (short)0, (int)tab_bytes, kRsrcWord3);
     // the host picks NP = ceil(nq / 64): only a lane's last pair can be past the row
     __builtin_assume(nq > kWave * (NP - 1) && nq <= kWave * NP);
-    const int64_t o0 = a.sent_off[s];
-    const int L = (int)(a.sent_off[s + 1] - o0);
+    const int64_t o0 = load_meta<NP, MF>(a.sent_off + s);
+    const int L = (int)(load_meta<NP, MF>(a.sent_off + s + 1) - o0);
     if (L <= 0) return;  // an empty sentence trains nothing (cpu:539-542, 575)
     const int32_t *sen = a.tokens + o0;
     const bool planned = a.plan_b != nullptr;
-    const float alpha = planned ? a.plan_alpha[2 * s + 1] : sentence_alpha(a.a0, a.epoch, a.T, a.iter, raw);
-    const float alpha0 = planned ? a.plan_alpha[2 * s] : alpha;  // centre 0 (cpu:431-447 timing)
+    const float alpha =
+        planned ? load_meta<NP, MF>(a.plan_alpha + 2 * s + 1) : sentence_alpha(a.a0, a.epoch, a.T, a.iter, raw);
+    const float alpha0 = planned ? load_meta<NP, MF>(a.plan_alpha + 2 * s) : alpha;  // centre 0 (cpu:431-447 timing)
     const uint32_t ep = (uint32_t)a.epoch;
     // rounding-noise stream of this lane for this sentence (never zero)
     uint32_t rs = hash32((uint32_t)a.seed ^ hash32((uint32_t)s * 0x632BE5ABu ^ (ep << 24)) ^ ((uint32_t)lane << 8)) | 1u;
@@ -1313,8 +1327,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2))) void s
             const int64_t idx1 = idx0 + grab < n ? idx0 + grab : n;
             for (int64_t idx = idx0; idx < idx1; idx++) {
                 const int64_t slot = a.s_begin + idx;
-                const int64_t s = a.order ? (int64_t)a.order[slot] : slot;
-                const int64_t raw = a.plan_b ? 0 : (a.order ? a.queue_raw[slot] : a.sent_raw[s]);
+                const int64_t s = a.order ? (int64_t)load_meta<NP, MF>(a.order + slot) : slot;
+                const int64_t raw =
+                    a.plan_b ? 0 : (a.order ? load_meta<NP, MF>(a.queue_raw + slot) : load_meta<NP, MF>(a.sent_raw + s));
                 SentCount c{0u, 0u, 0u, 0u, 0ull};
                 train_sentence<T, NP, NB, MF, WIN, WOUT>(a, s, raw, lane, lds, loss_a, loss_r, c);
                 fold(c);
