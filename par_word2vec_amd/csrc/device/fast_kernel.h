@@ -81,9 +81,11 @@ typedef __bf16 __attribute__((ext_vector_type(2))) bf16x2_t;
 // The packed bf16 step s - base from base to s = cur rounded to bf16 (0 if
 // the pair does not move).
 __device__ __forceinline__ uint32_t bf16_step_to(float2 base, float2 cur, uint32_t &rs, int nearest) {
-    rs ^= rs << 13;  // per-lane xorshift32 stream (Marsaglia 2003) for rounding noise
-    rs ^= rs >> 17;
-    rs ^= rs << 5;
+    // per-lane Weyl stream of rounding noise, seeded per sentence and lane by
+    // a hash: stochastic rounding needs each noise value uniform and
+    // independent of the value it rounds, which the golden-ratio step gives
+    // at one add (an xorshift32 step took six instructions)
+    rs += 0x9E3779B9u;
     // both roundings add a value below the kept 16 bits to the fp32 bit
     // pattern and truncate: 16 uniform bits for stochastic, half an ulp for
     // nearest (ties away from zero)
@@ -99,9 +101,7 @@ __device__ __forceinline__ uint32_t bf16_step_to(float2 base, float2 cur, uint32
 // cur rounded to bf16 (the same rounding and noise stream as bf16_step_to),
 // packed: x in the low half, y in the high half.
 __device__ __forceinline__ uint32_t bf16_round_to(float2 cur, uint32_t &rs, int nearest) {
-    rs ^= rs << 13;
-    rs ^= rs >> 17;
-    rs ^= rs << 5;
+    rs += 0x9E3779B9u;
     const uint32_t noise = nearest ? 0x80008000u : rs;
     const uint32_t bx = __builtin_bit_cast(uint32_t, cur.x), by = __builtin_bit_cast(uint32_t, cur.y);
     return __builtin_amdgcn_perm(by + (noise >> 16), bx + (noise & 0xFFFFu), 0x07060302u);
@@ -647,7 +647,8 @@ __device__ __forceinline__ void train_sentence(const FastArgs &a, int64_t s, int
         planned ? load_meta<NP, MF>(a.plan_alpha + 2 * s + 1) : sentence_alpha(a.a0, a.epoch, a.T, a.iter, raw);
     const float alpha0 = planned ? load_meta<NP, MF>(a.plan_alpha + 2 * s) : alpha;  // centre 0 (cpu:431-447 timing)
     const uint32_t ep = (uint32_t)a.epoch;
-    // rounding-noise stream of this lane for this sentence (never zero)
+    // start of this lane's rounding-noise stream for this sentence, hashed
+    // from the seed, the sentence, the epoch and the lane
     uint32_t rs = hash32((uint32_t)a.seed ^ hash32((uint32_t)s * 0x632BE5ABu ^ (ep << 24)) ^ ((uint32_t)lane << 8)) | 1u;
     // LDS of this wave: ring values (fp32), ring snapshots (storage type),
     // shared negatives
